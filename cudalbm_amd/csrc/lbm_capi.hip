@@ -1,0 +1,754 @@
+// lbm_capi.hip -- the C ABI of include/lbm_amd.h: handle lifecycle, layout
+// conversion, stepping through hipGraphs, fused function objects, FP64
+// reductions and the integer streaming probe.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_amd.h"
+#include "aux_kernels.hpp"
+#include "lbm_internal.hpp"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError {
+    hipError_t e;
+    const char *what;
+};
+
+#define HIPX(expr)                                               \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) throw HipError{e_, #expr};         \
+    } while (0)
+
+struct ArgError {
+    int code;
+    std::string msg;
+};
+
+// ------------------------------------------------------------ dispatch
+template <int Q, int TX, int TY>
+void launch_step_t(const StepParams &p, int grid, hipStream_t s, int storage, bool fo, bool probe) {
+    const dim3 g(grid), b(TX * TY);
+    if (probe)
+        hipLaunchKernelGGL((step_kernel<Q, TX, TY, float, false, true>), g, b, 0, s, p);
+    else if (storage == LBM_STORE_FP32) {
+        if (fo)
+            hipLaunchKernelGGL((step_kernel<Q, TX, TY, float, true, false>), g, b, 0, s, p);
+        else
+            hipLaunchKernelGGL((step_kernel<Q, TX, TY, float, false, false>), g, b, 0, s, p);
+    } else {
+        if (fo)
+            hipLaunchKernelGGL((step_kernel<Q, TX, TY, _Float16, true, false>), g, b, 0, s, p);
+        else
+            hipLaunchKernelGGL((step_kernel<Q, TX, TY, _Float16, false, false>), g, b, 0, s, p);
+    }
+}
+
+template <int Q, int TX, int TY>
+void launch_fill_t(const StepParams &p, int grid, hipStream_t s, int storage, bool probe) {
+    const dim3 g(grid), b(TX * TY);
+    if (probe)
+        hipLaunchKernelGGL((fill_faces_kernel<Q, TX, TY, float, true>), g, b, 0, s, p);
+    else if (storage == LBM_STORE_FP32)
+        hipLaunchKernelGGL((fill_faces_kernel<Q, TX, TY, float, false>), g, b, 0, s, p);
+    else
+        hipLaunchKernelGGL((fill_faces_kernel<Q, TX, TY, _Float16, false>), g, b, 0, s, p);
+}
+
+template <int Q>
+bool launch_step_q(int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool fo,
+                   bool probe) {
+    if (tx == 16 && ty == 16) launch_step_t<Q, 16, 16>(p, grid, s, storage, fo, probe);
+    else if (tx == 32 && ty == 8) launch_step_t<Q, 32, 8>(p, grid, s, storage, fo, probe);
+    else if (tx == 8 && ty == 8) launch_step_t<Q, 8, 8>(p, grid, s, storage, fo, probe);
+    else if (tx == 32 && ty == 16) launch_step_t<Q, 32, 16>(p, grid, s, storage, fo, probe);
+    else return false;
+    return true;
+}
+template <int Q>
+bool launch_fill_q(int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool probe) {
+    if (tx == 16 && ty == 16) launch_fill_t<Q, 16, 16>(p, grid, s, storage, probe);
+    else if (tx == 32 && ty == 8) launch_fill_t<Q, 32, 8>(p, grid, s, storage, probe);
+    else if (tx == 8 && ty == 8) launch_fill_t<Q, 8, 8>(p, grid, s, storage, probe);
+    else if (tx == 32 && ty == 16) launch_fill_t<Q, 32, 16>(p, grid, s, storage, probe);
+    else return false;
+    return true;
+}
+
+}  // namespace
+
+namespace lbm {
+
+bool tile_supported(int tx, int ty) {
+    return (tx == 16 && ty == 16) || (tx == 32 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 16);
+}
+
+void launch_step(int Q, int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool fo,
+                 bool probe) {
+    const bool ok = Q == 19 ? launch_step_q<19>(tx, ty, p, grid, s, storage, fo, probe)
+                            : launch_step_q<27>(tx, ty, p, grid, s, storage, fo, probe);
+    if (!ok) throw ArgError{LBM_ERR_ARG, "unsupported tile"};
+    HIPX(hipGetLastError());
+}
+
+void launch_fill(int Q, int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool probe) {
+    const bool ok = Q == 19 ? launch_fill_q<19>(tx, ty, p, grid, s, storage, probe)
+                            : launch_fill_q<27>(tx, ty, p, grid, s, storage, probe);
+    if (!ok) throw ArgError{LBM_ERR_ARG, "unsupported tile"};
+    HIPX(hipGetLastError());
+}
+
+int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_t *yoff, int64_t *zoff) {
+    const int QF = Q == 19 ? 5 : 9;
+    const int64_t nbx = nx / tx, nby = ny / ty, nc = nz / lz;
+    const int64_t xw = 2 * nbx * QF * (int64_t)nz * ny;
+    const int64_t yw = 2 * nby * QF * (int64_t)nz * nx;
+    const int64_t zw = 2 * nc * QF * (int64_t)ny * nx;
+    if (yoff) *yoff = xw;
+    if (zoff) *zoff = xw + yw;
+    return xw + yw + zw;
+}
+
+int choose_lz(int nx, int ny, int nz, int tx, int ty) {
+    const int64_t cols = (int64_t)(nx / tx) * (ny / ty);
+    int best = -1;
+    for (int lz = std::min(nz, 64); lz >= 1; lz--) {
+        if (nz % lz) continue;
+        if (cols * (nz / lz) >= 1024) return lz;
+        best = lz;  // smallest divisor seen so far
+    }
+    // not enough columns for 1024 workgroups: keep chunks of at least 4 planes
+    for (int lz = 4; lz <= nz; lz++)
+        if (nz % lz == 0) return lz;
+    return best > 0 ? best : nz;
+}
+
+void choose_tile(int nx, int ny, int *tx, int *ty) {
+    if (nx % 16 == 0 && ny % 16 == 0) { *tx = 16; *ty = 16; }
+    else if (nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
+    else { *tx = 8; *ty = 8; }
+}
+
+// Internal (column-tiled) index of a node.
+static inline int64_t internal_index(int x, int y, int z, int nx, int ny, int tx, int ty) {
+    const int nbx = nx / tx, nby = ny / ty;
+    const int bx = x / tx, by = y / ty;
+    return ((((int64_t)z * nby + by) * nbx + bx) * (tx * ty)) + (int64_t)(y % ty) * tx + (x % tx);
+}
+
+// perm[u] = internal index of the node at user-layout position u.
+std::vector<int64_t> make_perm(int layout, int nx, int ny, int nz, int tx, int ty) {
+    std::vector<int64_t> perm((size_t)nx * ny * nz);
+    if (layout == LBM_LAYOUT_GLOBAL) {
+        size_t u = 0;
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) perm[u++] = internal_index(x, y, z, nx, ny, tx, ty);
+    } else {
+        // reference 8^3 tiles: tx + 8(ty + 8(tz + 8(bx + NBX(by + NBY bz))))
+        const int NBX = nx / 8, NBY = ny / 8, NBZ = nz / 8;
+        size_t u = 0;
+        for (int bz = 0; bz < NBZ; bz++)
+            for (int by = 0; by < NBY; by++)
+                for (int bx = 0; bx < NBX; bx++)
+                    for (int tz = 0; tz < 8; tz++)
+                        for (int ty8 = 0; ty8 < 8; ty8++)
+                            for (int tx8 = 0; tx8 < 8; tx8++)
+                                perm[u++] = internal_index(bx * 8 + tx8, by * 8 + ty8, bz * 8 + tz, nx, ny, tx, ty);
+    }
+    return perm;
+}
+
+}  // namespace lbm
+
+struct lbm_handle {
+    lbm_config cfg{};
+    int Q = 19, TX = 16, TY = 16, LZ = 16, nbx = 0, nby = 0, nc = 0;
+    int64_t n = 0;
+    double omega = 0, tau = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    void *mom[10] = {};
+    float *faces[2] = {nullptr, nullptr};
+    int64_t fwords = 0, yoff = 0, zoff = 0;
+    int parity = 0;
+    unsigned fo_mask = 0;
+    float *k = nullptr, *S = nullptr;
+    double *kmean = nullptr, *smean = nullptr;
+    int64_t *tcount = nullptr;
+    int64_t t = 0;
+    float *uprev = nullptr;
+    double *part = nullptr, *red_out = nullptr;
+    int red_blocks = 1024;
+    bool red_first = true;
+    std::map<int, hipGraphExec_t> graphs;  // key: parity*2 + fo flag, for kGraphSteps steps
+    std::map<int, std::vector<int64_t>> perms;
+    std::string err;
+
+    const std::vector<int64_t> &perm(int layout) {
+        auto it = perms.find(layout);
+        if (it == perms.end()) it = perms.emplace(layout, make_perm(layout, cfg.nx, cfg.ny, cfg.nz, TX, TY)).first;
+        return it->second;
+    }
+
+    StepParams params(bool probe = false) const {
+        StepParams p{};
+        p.nx = cfg.nx;
+        p.ny = cfg.ny;
+        p.nz = cfg.nz;
+        p.lz = LZ;
+        p.nbx = nbx;
+        p.nby = nby;
+        p.nc = nc;
+        p.omega = (float)omega;
+        p.t_omega = (float)(1.0 - omega);
+        p.omega_h = (float)(omega * 0.5);
+        p.u_lid = (float)cfg.u_inf;
+        p.tau = (float)tau;
+        for (int i = 0; i < 10; i++) p.mom[i] = mom[i];
+        p.fin = faces[parity];
+        p.fout = faces[parity ^ 1];
+        p.yoff = yoff;
+        p.zoff = zoff;
+        codec(p.enc, p.dec);
+        p.fo_mask = probe ? 0u : fo_mask;
+        p.k = k;
+        p.kmean = kmean;
+        p.S = S;
+        p.smean = smean;
+        p.tcount = tcount;
+        p.n = n;
+        return p;
+    }
+    void codec(float *enc, float *dec) const {
+        // FP16 storage: (rho-1)/cs2, u/cs, (Pi_ab/rho - cs2 d_ab)/cs2
+        static const double e[10] = {3.0, 1.0 / std::sqrt(3.0), 1.0 / std::sqrt(3.0), 1.0 / std::sqrt(3.0),
+                                     2.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0};
+        for (int i = 0; i < 10; i++) {
+            const double v = cfg.storage == LBM_STORE_FP16 ? e[i] : 1.0;
+            enc[i] = (float)v;
+            dec[i] = (float)(1.0 / v);
+        }
+    }
+    int grid() const { return nbx * nby * nc; }
+    size_t elem() const { return cfg.storage == LBM_STORE_FP16 ? 2 : 4; }
+
+    void drop_graphs() {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
+    void release() {
+        drop_graphs();
+        for (auto &m : mom) {
+            if (m) (void)hipFree(m);
+            m = nullptr;
+        }
+        for (auto &f : faces) {
+            if (f) (void)hipFree(f);
+            f = nullptr;
+        }
+        for (void *p : {(void *)k, (void *)S, (void *)kmean, (void *)smean, (void *)tcount, (void *)uprev,
+                        (void *)part, (void *)red_out})
+            if (p) (void)hipFree(p);
+        k = S = nullptr;
+        kmean = smean = nullptr;
+        tcount = nullptr;
+        uprev = nullptr;
+        part = red_out = nullptr;
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+namespace {
+
+constexpr int kGraphSteps = 64;
+
+int fail(lbm_handle *h, const HipError &e) {
+    if (h) h->err = std::string("HIP error ") + hipGetErrorString(e.e) + " in " + e.what;
+    return LBM_ERR_HIP;
+}
+int fail(lbm_handle *h, const ArgError &e) {
+    if (h) h->err = e.msg;
+    return e.code;
+}
+
+template <typename F>
+int guarded(lbm_handle *h, F &&f) {
+    if (!h) return LBM_ERR_ARG;
+    try {
+        HIPX(hipSetDevice(h->cfg.device));
+        f();
+        return LBM_OK;
+    } catch (const HipError &e) {
+        return fail(h, e);
+    } catch (const ArgError &e) {
+        return fail(h, e);
+    } catch (const std::bad_alloc &) {
+        h->err = "host allocation failed";
+        return LBM_ERR_NOMEM;
+    } catch (...) {
+        h->err = "unexpected exception";
+        return LBM_ERR_ARG;
+    }
+}
+
+void upload_moment(lbm_handle *h, int k, const float *src, const std::vector<int64_t> &perm) {
+    std::vector<float> tmp((size_t)h->n);
+    for (int64_t u = 0; u < h->n; u++) tmp[(size_t)perm[(size_t)u]] = src[u];
+    if (h->cfg.storage == LBM_STORE_FP32) {
+        HIPX(hipMemcpyAsync(h->mom[k], tmp.data(), sizeof(float) * h->n, hipMemcpyHostToDevice, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+    } else {
+        float *stage = nullptr;
+        HIPX(hipMalloc(&stage, sizeof(float) * h->n));
+        float enc[10], dec[10];
+        h->codec(enc, dec);
+        hipError_t e = hipMemcpyAsync(stage, tmp.data(), sizeof(float) * h->n, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(encode_f16_kernel, dim3(2048), dim3(256), 0, h->stream, stage,
+                               (_Float16 *)h->mom[k], h->n, enc[k]);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        (void)hipFree(stage);
+        HIPX(e);
+    }
+}
+
+void download_moment(lbm_handle *h, int k, float *dst, const std::vector<int64_t> &perm) {
+    std::vector<float> tmp((size_t)h->n);
+    if (h->cfg.storage == LBM_STORE_FP32) {
+        HIPX(hipMemcpyAsync(tmp.data(), h->mom[k], sizeof(float) * h->n, hipMemcpyDeviceToHost, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+    } else {
+        float *stage = nullptr;
+        HIPX(hipMalloc(&stage, sizeof(float) * h->n));
+        float enc[10], dec[10];
+        h->codec(enc, dec);
+        hipLaunchKernelGGL(decode_f16_kernel, dim3(2048), dim3(256), 0, h->stream, (const _Float16 *)h->mom[k],
+                           stage, h->n, dec[k]);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(tmp.data(), stage, sizeof(float) * h->n, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        (void)hipFree(stage);
+        HIPX(e);
+    }
+    for (int64_t u = 0; u < h->n; u++) dst[u] = tmp[(size_t)perm[(size_t)u]];
+}
+
+void rebuild_faces(lbm_handle *h) {
+    StepParams p = h->params();
+    p.fout = h->faces[h->parity];  // the set the next step reads
+    launch_fill(h->Q, h->TX, h->TY, p, h->grid(), h->stream, h->cfg.storage, false);
+}
+
+void check_layout(lbm_handle *h, int layout) {
+    if (layout != LBM_LAYOUT_GLOBAL && layout != LBM_LAYOUT_TILED8) throw ArgError{LBM_ERR_ARG, "bad layout"};
+    if (layout == LBM_LAYOUT_TILED8 && (h->cfg.nx % 8 || h->cfg.ny % 8 || h->cfg.nz % 8))
+        throw ArgError{LBM_ERR_SIZE, "TILED8 layout needs nx, ny, nz multiples of 8"};
+}
+
+// One step, launched on the handle's stream (captured or not).
+void enqueue_step(lbm_handle *h, int parity) {
+    StepParams p = h->params();
+    p.fin = h->faces[parity];
+    p.fout = h->faces[parity ^ 1];
+    const bool fo = h->fo_mask != 0;
+    launch_step(h->Q, h->TX, h->TY, p, h->grid(), h->stream, h->cfg.storage, fo, false);
+    if (h->fo_mask & (LBM_FO_KMEAN | LBM_FO_SMEAN)) {
+        hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, h->stream, h->tcount);
+        HIPX(hipGetLastError());
+    }
+}
+
+hipGraphExec_t graph_for(lbm_handle *h) {
+    const int key = h->parity * 2 + (h->fo_mask ? 1 : 0);
+    auto it = h->graphs.find(key);
+    if (it != h->graphs.end()) return it->second;
+    hipGraph_t g = nullptr;
+    HIPX(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int i = 0; i < kGraphSteps; i++) enqueue_step(h, h->parity ^ (i & 1));
+    } catch (...) {
+        hipGraph_t dummy = nullptr;
+        (void)hipStreamEndCapture(h->stream, &dummy);
+        if (dummy) (void)hipGraphDestroy(dummy);
+        throw;
+    }
+    HIPX(hipStreamEndCapture(h->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPX(e);
+    h->graphs[key] = ge;
+    return ge;
+}
+
+void do_steps(lbm_handle *h, int n) {
+    int left = n;
+    while (left >= kGraphSteps) {
+        hipGraphExec_t ge = graph_for(h);
+        HIPX(hipGraphLaunch(ge, h->stream));
+        left -= kGraphSteps;  // even count: parity unchanged
+    }
+    for (int i = 0; i < left; i++) {
+        enqueue_step(h, h->parity);
+        h->parity ^= 1;
+    }
+    h->t += n;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_create_error(void) { return g_create_error.c_str(); }
+
+int lbm_create(const lbm_config *cfg, lbm_handle **out) {
+    g_create_error.clear();
+    if (!cfg || !out) {
+        g_create_error = "null argument";
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    lbm_handle *h = new (std::nothrow) lbm_handle();
+    if (!h) return LBM_ERR_NOMEM;
+    h->cfg = *cfg;
+    const lbm_config &c = h->cfg;
+    auto bad = [&](int code, const std::string &msg) {
+        g_create_error = msg;
+        h->release();
+        delete h;
+        return code;
+    };
+    if (c.velocity_set != 19 && c.velocity_set != 27) return bad(LBM_ERR_ARG, "velocity_set must be 19 or 27");
+    if (c.storage != LBM_STORE_FP32 && c.storage != LBM_STORE_FP16) return bad(LBM_ERR_ARG, "bad storage");
+    if (c.nx < 2 || c.ny < 2 || c.nz < 2) return bad(LBM_ERR_SIZE, "mesh dimensions must be >= 2");
+    if (!(c.Re > 0) || !std::isfinite(c.u_inf)) return bad(LBM_ERR_ARG, "Re must be > 0 and u_inf finite");
+    h->Q = c.velocity_set;
+    if (c.tile_x == 0 || c.tile_y == 0)
+        choose_tile(c.nx, c.ny, &h->TX, &h->TY);
+    else {
+        h->TX = c.tile_x;
+        h->TY = c.tile_y;
+    }
+    if (!tile_supported(h->TX, h->TY)) return bad(LBM_ERR_ARG, "unsupported tile footprint");
+    if (c.nx % h->TX || c.ny % h->TY)
+        return bad(LBM_ERR_SIZE, "nx and ny must be multiples of the tile footprint " + std::to_string(h->TX) +
+                                     "x" + std::to_string(h->TY));
+    h->LZ = c.tile_z > 0 ? c.tile_z : choose_lz(c.nx, c.ny, c.nz, h->TX, h->TY);
+    if (c.nz % h->LZ) return bad(LBM_ERR_SIZE, "nz must be a multiple of tile_z");
+    h->nbx = c.nx / h->TX;
+    h->nby = c.ny / h->TY;
+    h->nc = c.nz / h->LZ;
+    h->n = (int64_t)c.nx * c.ny * c.nz;
+    if (h->n * 32 + 31 >= (int64_t)UINT32_MAX) return bad(LBM_ERR_SIZE, "mesh too large for 32-bit probe ids");
+    // latticeMesh.cuh:178-182 (computed in FP64 here)
+    const double nu = c.u_inf * (double)(c.nx - 1) / c.Re;
+    h->tau = 0.5 + 3.0 * nu;
+    h->omega = 1.0 / h->tau;
+    try {
+        HIPX(hipSetDevice(c.device));
+        if (c.stream)
+            h->stream = (hipStream_t)c.stream;
+        else {
+            HIPX(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+            h->own_stream = true;
+        }
+        for (int i = 0; i < 10; i++) {
+            HIPX(hipMalloc(&h->mom[i], h->elem() * h->n));
+            HIPX(hipMemsetAsync(h->mom[i], 0, h->elem() * h->n, h->stream));
+        }
+        h->fwords = face_words(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY, h->LZ, &h->yoff, &h->zoff);
+        for (int s = 0; s < 2; s++) {
+            HIPX(hipMalloc(&h->faces[s], sizeof(float) * h->fwords));
+            HIPX(hipMemsetAsync(h->faces[s], 0, sizeof(float) * h->fwords, h->stream));
+        }
+        HIPX(hipMalloc(&h->tcount, sizeof(int64_t)));
+        HIPX(hipMemsetAsync(h->tcount, 0, sizeof(int64_t), h->stream));
+        rebuild_faces(h);
+        HIPX(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return bad(LBM_ERR_HIP, std::string("HIP error ") + hipGetErrorString(e.e) + " in " + e.what);
+    } catch (const ArgError &e) {
+        return bad(e.code, e.msg);
+    }
+    *out = h;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_handle *h) {
+    if (!h) return LBM_ERR_ARG;
+    (void)hipSetDevice(h->cfg.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->release();
+    delete h;
+    return LBM_OK;
+}
+
+const char *lbm_last_error(const lbm_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int lbm_get_tiling(const lbm_handle *h, int *tx, int *ty, int *tz) {
+    if (!h) return LBM_ERR_ARG;
+    if (tx) *tx = h->TX;
+    if (ty) *ty = h->TY;
+    if (tz) *tz = h->LZ;
+    return LBM_OK;
+}
+
+int lbm_get_omega(const lbm_handle *h, double *omega, double *tau) {
+    if (!h) return LBM_ERR_ARG;
+    if (omega) *omega = h->omega;
+    if (tau) *tau = h->tau;
+    return LBM_OK;
+}
+
+int lbm_set_moments(lbm_handle *h, const float *const m[10], int layout) {
+    return guarded(h, [&] {
+        if (!m) throw ArgError{LBM_ERR_ARG, "null moment array"};
+        check_layout(h, layout);
+        HIPX(hipStreamSynchronize(h->stream));
+        const auto &perm = h->perm(layout);
+        for (int k = 0; k < 10; k++) {
+            if (!m[k]) throw ArgError{LBM_ERR_ARG, "null moment array"};
+            upload_moment(h, k, m[k], perm);
+        }
+        rebuild_faces(h);
+        HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+int lbm_get_moments(lbm_handle *h, float *m[10], int layout) {
+    return guarded(h, [&] {
+        if (!m) throw ArgError{LBM_ERR_ARG, "null moment array"};
+        check_layout(h, layout);
+        HIPX(hipStreamSynchronize(h->stream));
+        const auto &perm = h->perm(layout);
+        for (int k = 0; k < 10; k++) {
+            if (!m[k]) throw ArgError{LBM_ERR_ARG, "null moment array"};
+            download_moment(h, k, m[k], perm);
+        }
+    });
+}
+
+int lbm_set_moments_f64(lbm_handle *h, const double *const m[10], int layout) {
+    if (!h || !m) return LBM_ERR_ARG;
+    std::vector<std::vector<float>> tmp(10);
+    const float *ptrs[10];
+    for (int k = 0; k < 10; k++) {
+        if (!m[k]) return LBM_ERR_ARG;
+        tmp[k].resize((size_t)h->n);
+        for (int64_t i = 0; i < h->n; i++) tmp[k][(size_t)i] = (float)m[k][i];
+        ptrs[k] = tmp[k].data();
+    }
+    return lbm_set_moments(h, ptrs, layout);
+}
+
+int lbm_get_moments_f64(lbm_handle *h, double *m[10], int layout) {
+    if (!h || !m) return LBM_ERR_ARG;
+    std::vector<std::vector<float>> tmp(10);
+    float *ptrs[10];
+    for (int k = 0; k < 10; k++) {
+        tmp[k].resize((size_t)h->n);
+        ptrs[k] = tmp[k].data();
+    }
+    const int rc = lbm_get_moments(h, ptrs, layout);
+    if (rc) return rc;
+    for (int k = 0; k < 10; k++) {
+        if (!m[k]) return LBM_ERR_ARG;
+        for (int64_t i = 0; i < h->n; i++) m[k][i] = tmp[k][(size_t)i];
+    }
+    return LBM_OK;
+}
+
+int lbm_step(lbm_handle *h, int n) {
+    return guarded(h, [&] {
+        if (n < 0) throw ArgError{LBM_ERR_ARG, "negative step count"};
+        do_steps(h, n);
+    });
+}
+
+int lbm_synchronize(lbm_handle *h) {
+    return guarded(h, [&] { HIPX(hipStreamSynchronize(h->stream)); });
+}
+
+int lbm_step_timed(lbm_handle *h, int n, double *ms) {
+    return guarded(h, [&] {
+        if (n < 0 || !ms) throw ArgError{LBM_ERR_ARG, "bad argument"};
+        // build graphs outside the timed region
+        if (n >= kGraphSteps) {
+            (void)graph_for(h);
+        }
+        hipEvent_t a, b;
+        HIPX(hipEventCreate(&a));
+        HIPX(hipEventCreate(&b));
+        hipError_t e = hipEventRecord(a, h->stream);
+        try {
+            if (e == hipSuccess) do_steps(h, n);
+        } catch (...) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            throw;
+        }
+        if (e == hipSuccess) e = hipEventRecord(b, h->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float f = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        HIPX(e);
+        *ms = f;
+    });
+}
+
+int lbm_get_time(const lbm_handle *h, int64_t *t) {
+    if (!h || !t) return LBM_ERR_ARG;
+    *t = h->t;
+    return LBM_OK;
+}
+
+int lbm_set_time(lbm_handle *h, int64_t t) {
+    return guarded(h, [&] {
+        if (t < 0) throw ArgError{LBM_ERR_ARG, "negative time"};
+        h->t = t;
+        HIPX(hipMemcpyAsync(h->tcount, &h->t, sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+int lbm_fo_enable(lbm_handle *h, unsigned mask) {
+    return guarded(h, [&] {
+        if (mask & ~15u) throw ArgError{LBM_ERR_ARG, "unknown function-object bits"};
+        HIPX(hipStreamSynchronize(h->stream));
+        h->drop_graphs();
+        const int64_t n = h->n;
+        if ((mask & LBM_FO_K) && !h->k) HIPX(hipMalloc(&h->k, sizeof(float) * n));
+        if ((mask & LBM_FO_S) && !h->S) HIPX(hipMalloc(&h->S, sizeof(float) * 6 * n));
+        if ((mask & LBM_FO_KMEAN) && !h->kmean) HIPX(hipMalloc(&h->kmean, sizeof(double) * n));
+        if ((mask & LBM_FO_SMEAN) && !h->smean) HIPX(hipMalloc(&h->smean, sizeof(double) * 6 * n));
+        if (mask & LBM_FO_KMEAN) {
+            hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, h->kmean, n, 0.0);
+            HIPX(hipGetLastError());
+        }
+        if (mask & LBM_FO_SMEAN) {
+            hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, h->smean, 6 * n, 0.0);
+            HIPX(hipGetLastError());
+        }
+        HIPX(hipMemcpyAsync(h->tcount, &h->t, sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+        h->fo_mask = mask;
+    });
+}
+
+int lbm_fo_get(lbm_handle *h, int field, double *out, int layout) {
+    return guarded(h, [&] {
+        if (!out) throw ArgError{LBM_ERR_ARG, "null output"};
+        check_layout(h, layout);
+        const unsigned need = field == LBM_FIELD_K ? LBM_FO_K
+                              : field == LBM_FIELD_KMEAN ? LBM_FO_KMEAN
+                              : field == LBM_FIELD_S ? LBM_FO_S
+                              : field == LBM_FIELD_SMEAN ? LBM_FO_SMEAN : 0u;
+        if (!need) throw ArgError{LBM_ERR_ARG, "unknown field"};
+        if (!(h->fo_mask & need)) throw ArgError{LBM_ERR_STATE, "function object not enabled"};
+        HIPX(hipStreamSynchronize(h->stream));
+        const int ncomp = (field == LBM_FIELD_S || field == LBM_FIELD_SMEAN) ? 6 : 1;
+        const int64_t n = h->n;
+        const auto &perm = h->perm(layout);
+        std::vector<double> tmp((size_t)(n * ncomp));
+        if (field == LBM_FIELD_K || field == LBM_FIELD_S) {
+            std::vector<float> f((size_t)(n * ncomp));
+            HIPX(hipMemcpy(f.data(), field == LBM_FIELD_K ? h->k : h->S, sizeof(float) * n * ncomp,
+                           hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < f.size(); i++) tmp[i] = f[i];
+        } else {
+            HIPX(hipMemcpy(tmp.data(), field == LBM_FIELD_KMEAN ? h->kmean : h->smean, sizeof(double) * n * ncomp,
+                           hipMemcpyDeviceToHost));
+        }
+        for (int c = 0; c < ncomp; c++)
+            for (int64_t u = 0; u < n; u++) out[c * n + u] = tmp[(size_t)(c * n + perm[(size_t)u])];
+    });
+}
+
+int lbm_reduce(lbm_handle *h, double *ke, double *res) {
+    return guarded(h, [&] {
+        if (!h->uprev) {
+            HIPX(hipMalloc(&h->uprev, sizeof(float) * 3 * h->n));
+            HIPX(hipMalloc(&h->part, sizeof(double) * 3 * h->red_blocks));
+            HIPX(hipMalloc(&h->red_out, sizeof(double) * 3));
+            h->red_first = true;
+        }
+        StepParams p = h->params();
+        constexpr int NT = 256;
+        if (h->cfg.storage == LBM_STORE_FP32)
+            hipLaunchKernelGGL((residual_partials_kernel<float, NT>), dim3(h->red_blocks), dim3(NT), 0, h->stream, p,
+                               h->uprev, h->part, h->red_first ? 1 : 0);
+        else
+            hipLaunchKernelGGL((residual_partials_kernel<_Float16, NT>), dim3(h->red_blocks), dim3(NT), 0,
+                               h->stream, p, h->uprev, h->part, h->red_first ? 1 : 0);
+        HIPX(hipGetLastError());
+        hipLaunchKernelGGL((residual_final_kernel<NT>), dim3(1), dim3(NT), 0, h->stream, h->part, h->red_blocks,
+                           h->red_out);
+        HIPX(hipGetLastError());
+        double r[3];
+        HIPX(hipMemcpyAsync(r, h->red_out, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+        h->red_first = false;
+        if (ke) *ke = r[0];
+        if (res) *res = r[2] > 0 ? std::sqrt(r[1] / r[2]) : (r[1] > 0 ? INFINITY : 0.0);
+    });
+}
+
+int lbm_probe_streaming(lbm_handle *h, uint32_t *src_out, uint8_t *type_out) {
+    return guarded(h, [&] {
+        if (!src_out || !type_out) throw ArgError{LBM_ERR_ARG, "null output"};
+        HIPX(hipStreamSynchronize(h->stream));
+        const int64_t n = h->n;
+        uint32_t *dsrc = nullptr;
+        uint8_t *dtype = nullptr;
+        HIPX(hipMalloc(&dsrc, sizeof(uint32_t) * n * h->Q));
+        hipError_t e = hipMalloc(&dtype, n);
+        if (e != hipSuccess) {
+            (void)hipFree(dsrc);
+            HIPX(e);
+        }
+        try {
+            StepParams p = h->params(true);
+            p.probe_src = dsrc;
+            p.probe_type = dtype;
+            // ids into the set the probe step reads, then one probe step
+            StepParams pf = p;
+            pf.fout = h->faces[h->parity];
+            launch_fill(h->Q, h->TX, h->TY, pf, h->grid(), h->stream, LBM_STORE_FP32, true);
+            launch_step(h->Q, h->TX, h->TY, p, h->grid(), h->stream, LBM_STORE_FP32, false, true);
+            HIPX(hipMemcpyAsync(src_out, dsrc, sizeof(uint32_t) * n * h->Q, hipMemcpyDeviceToHost, h->stream));
+            HIPX(hipMemcpyAsync(type_out, dtype, n, hipMemcpyDeviceToHost, h->stream));
+            HIPX(hipStreamSynchronize(h->stream));
+            // restore the face set from the moments
+            rebuild_faces(h);
+            HIPX(hipStreamSynchronize(h->stream));
+        } catch (...) {
+            (void)hipFree(dsrc);
+            (void)hipFree(dtype);
+            throw;
+        }
+        (void)hipFree(dsrc);
+        (void)hipFree(dtype);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,495 @@
+// step_kernel.hpp -- the fused moment-representation LBM step for gfx950.
+//
+// One workgroup owns a TX x TY column footprint of the domain and marches
+// through `lz` consecutive z-planes of it (a "chunk").  Per plane it
+//   1. holds the old moments of planes z and z+1 in registers (z+2 is in flight),
+//   2. rebuilds from them the populations that stream in-plane (cz = 0), down
+//      (cz = -1, from plane z+1) and -- kept from the previous plane -- up
+//      (cz = +1, from plane z-1), and stages them in LDS,
+//   3. fills a one-node LDS ring around the footprint from the neighbour
+//      columns' face buffers (written in the previous step),
+//   4. gathers every population from its source node x - c_q,
+//   5. forms moments (interior) or applies the wall/lid conditions (boundary),
+//      collides, writes the 10 moments back in place,
+//   6. writes the outgoing populations of its footprint edges to the other
+//      face-buffer set (x/y faces every plane, z faces at the chunk ends).
+// The face buffers therefore cost 4*QF*4 bytes*(1/TX + 1/TY + 1/lz) per node
+// instead of the 8^3-tile scheme's 4*QF*4*(3/8) (reference halo.cuh:171-494).
+//
+// Semantics follow the reference step (momentBasedD3Q19.cuh:117-184): pull
+// streaming f_q(x) = f*_q(x - c_q) of post-collision populations rebuilt from
+// the stored moments, boundary moments from the known populations, scale,
+// collide, store.
+#pragma once
+
+#include <hip/hip_fp16.h>
+
+#include "lattice.hpp"
+
+namespace lbm {
+
+// Barrier that orders LDS traffic only: global loads issued for the next plane
+// stay in flight across it (a plain __syncthreads would drain them).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+struct StepParams {
+    int nx, ny, nz, lz, nbx, nby, nc;
+    float omega, t_omega, omega_h, u_lid, tau;
+    void *mom[10];         // stored moments, column-tiled internal layout, in place
+    const float *fin;      // face buffers read (previous step): X | Y | Z
+    float *fout;           // face buffers written (this step)
+    int64_t yoff, zoff;    // offsets of the Y and Z face blocks inside one face set
+    float enc[10], dec[10];  // FP16 storage codec: stored16 = m*enc, m = stored16*dec
+    // fused function objects
+    unsigned fo_mask;
+    float *k;
+    double *kmean;
+    float *S;              // 6 component arrays, each of n floats
+    double *smean;         // 6 component arrays, each of n doubles
+    const int64_t *tcount; // absolute step index of this step (device counter)
+    int64_t n;             // nodes
+    // probe outputs
+    uint32_t *probe_src;
+    uint8_t *probe_type;
+};
+
+template <typename ST>
+struct Store;
+template <>
+struct Store<float> {
+    __device__ static __forceinline__ float load(const void *p, int64_t i, float) {
+        return __builtin_nontemporal_load(static_cast<const float *>(p) + i);
+    }
+    __device__ static __forceinline__ void store(void *p, int64_t i, float v, float) {
+        __builtin_nontemporal_store(v, static_cast<float *>(p) + i);
+    }
+};
+template <>
+struct Store<_Float16> {
+    __device__ static __forceinline__ float load(const void *p, int64_t i, float dec) {
+        return float(static_cast<const _Float16 *>(p)[i]) * dec;
+    }
+    __device__ static __forceinline__ void store(void *p, int64_t i, float v, float enc) {
+        static_cast<_Float16 *>(p)[i] = _Float16(v * enc);
+    }
+};
+
+// Probe payload: 1 + 32*(global x-fastest node index) + q, carried as float bits.
+__device__ __forceinline__ float probe_id(int x, int y, int z, int nx, int ny, int q) {
+    const uint32_t s = uint32_t(x) + uint32_t(nx) * (uint32_t(y) + uint32_t(ny) * uint32_t(z));
+    return __uint_as_float(1u + 32u * s + uint32_t(q));
+}
+
+template <int TX, int TY>
+struct Geo {
+    static constexpr int NT = TX * TY;
+    static constexpr int WP = TX + 2;            // LDS row pitch (with ring)
+    static constexpr int P = (TX + 2) * (TY + 2);  // LDS plane size
+};
+
+// Ring-fill jobs: every ring cell population that points into the footprint.
+// Sides: 0 W (ring x=-1, cx=+1 pops, from the east faces of column bx-1),
+//        1 E (ring x=TX, cx=-1, west faces of column bx+1),
+//        2 S (ring y=-1, cy=+1, north faces of row by-1; corners excluded),
+//        3 N (ring y=TY, cy=-1, south faces of row by+1; corners excluded).
+template <int Q, int TX, int TY>
+struct Ring {
+    static constexpr int QF = VS<Q>::QF;
+    __host__ __device__ static constexpr int side_count(int side, int k) {
+        if (side < 2) return TY;
+        const int q = face_q<Q>(1, side == 2 ? 1 : -1, k);
+        return TX - absi(cxq(q));
+    }
+    __host__ __device__ static constexpr int total() {
+        int t = 0;
+        for (int s = 0; s < 4; s++)
+            for (int k = 0; k < QF; k++) t += side_count(s, k);
+        return t;
+    }
+    static constexpr int NJ = total();
+    static constexpr int JPT = (NJ + TX * TY - 1) / (TX * TY);
+};
+
+struct Job {
+    int lds;      // LDS word offset, or -1 when the thread has no job
+    int64_t gb;   // face-set offset without the plane term, or -1 when outside the domain
+    int zs;       // plane stride inside the face block
+    int dz;       // source plane offset: -cz
+};
+
+template <int Q, int TX, int TY>
+__device__ __forceinline__ Job make_job(int j, const StepParams &p, int bx, int by) {
+    using R = Ring<Q, TX, TY>;
+    using G = Geo<TX, TY>;
+    constexpr int QF = VS<Q>::QF;
+    Job job{-1, -1, 0, 0};
+    if (j >= R::NJ) return job;
+    int side = 0, k = 0, pos = j;
+    // locate (side, k, pos)
+    for (int s = 0; s < 4; s++) {
+        bool found = false;
+        for (int kk = 0; kk < QF; kk++) {
+            const int c = R::side_count(s, kk);
+            if (pos < c) {
+                side = s;
+                k = kk;
+                found = true;
+                break;
+            }
+            pos -= c;
+        }
+        if (found) break;
+    }
+    const int axis = side < 2 ? 0 : 1;
+    const int sign = (side == 0 || side == 2) ? 1 : -1;
+    // population index, by a runtime scan of the compile-time table
+    int q = 0;
+    for (int kk = 0; kk < QF; kk++)
+        if (kk == k) q = face_q<Q>(axis, sign, kk);
+    // the population table is small; recompute its components at run time
+    int qcx = 0, qcy = 0, qcz = 0;
+    for (int qq = 0; qq < Q; qq++)
+        if (qq == q) {
+            qcx = kC[qq][0];
+            qcy = kC[qq][1];
+            qcz = kC[qq][2];
+        }
+    int rx, ry;
+    if (side < 2) {
+        rx = side == 0 ? -1 : TX;
+        ry = pos - qcy;  // ry in [-cy, TY-1-cy]
+    } else {
+        ry = side == 2 ? -1 : TY;
+        rx = pos + (qcx < 0 ? 1 : 0);  // rx in [max(0,-cx), min(TX-1, TX-1-cx)]
+    }
+    const int slot = q <= 4 ? q - 1 : q - 3;
+    job.lds = slot * G::P + (ry + 1) * G::WP + (rx + 1);
+    job.dz = -qcz;
+    const int gx = bx * TX + rx, gy = by * TY + ry;
+    const bool inside = gx >= 0 && gx < p.nx && gy >= 0 && gy < p.ny;
+    if (side < 2) {
+        const int col = side == 0 ? bx - 1 : bx + 1;
+        const int s_face = side == 0 ? 1 : 0;  // read the east (1) / west (0) faces
+        job.zs = p.ny;
+        job.gb = inside ? ((((int64_t)s_face * p.nbx + col) * QF + k) * p.nz) * p.ny + gy : -1;
+    } else {
+        const int row = side == 2 ? by - 1 : by + 1;
+        const int s_face = side == 2 ? 1 : 0;
+        job.zs = p.nx;
+        job.gb = inside ? p.yoff + ((((int64_t)s_face * p.nby + row) * QF + k) * p.nz) * p.nx + gx : -1;
+    }
+    return job;
+}
+
+__device__ __forceinline__ float job_load(const Job &j, const float *__restrict__ fin, int zdest, int nz) {
+    const int zs = zdest + j.dz;
+    if (j.gb < 0 || zs < 0 || zs >= nz) return 0.f;
+    return fin[j.gb + (int64_t)zs * j.zs];
+}
+
+// Offsets of one population inside the face blocks.
+template <int Q>
+__device__ __forceinline__ int64_t xf_off(const StepParams &p, int s, int bx, int k, int z, int y) {
+    return ((((int64_t)s * p.nbx + bx) * VS<Q>::QF + k) * p.nz + z) * p.ny + y;
+}
+template <int Q>
+__device__ __forceinline__ int64_t yf_off(const StepParams &p, int s, int by, int k, int z, int x) {
+    return p.yoff + ((((int64_t)s * p.nby + by) * VS<Q>::QF + k) * p.nz + z) * p.nx + x;
+}
+template <int Q>
+__device__ __forceinline__ int64_t zf_off(const StepParams &p, int s, int c, int k, int y, int x) {
+    return p.zoff + ((((int64_t)s * p.nc + c) * VS<Q>::QF + k) * p.ny + y) * p.nx + x;
+}
+
+// Population q of node (x,y,z) rebuilt from its moments (or its probe id).
+template <int Q, int q, bool PROBE>
+__device__ __forceinline__ float pop_of(const Rec &r, const float (&m)[10], int x, int y, int z,
+                                        const StepParams &p) {
+    if constexpr (PROBE)
+        return probe_id(x, y, z, p.nx, p.ny, q);
+    else
+        return rec_dev<Q, q>(r, m);
+}
+
+// Outgoing face populations of one node after collision (or its probe ids).
+template <int Q, int TX, int TY, bool PROBE>
+__device__ __forceinline__ void write_faces(const StepParams &p, float *__restrict__ fout,
+                                            const float (&m)[10], int tx, int ty, int x, int y,
+                                            int z, int bx, int by, int c, int z0, int z1) {
+    constexpr int QF = VS<Q>::QF;
+    const Rec r = rec_setup(m);
+    if (tx == 0 && bx > 0)
+        unroll<0, QF>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int q = face_q<Q>(0, -1, k);
+            fout[xf_off<Q>(p, 0, bx, k, z, y)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+        });
+    if (tx == TX - 1 && bx < p.nbx - 1)
+        unroll<0, QF>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int q = face_q<Q>(0, 1, k);
+            fout[xf_off<Q>(p, 1, bx, k, z, y)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+        });
+    if (ty == 0 && by > 0)
+        unroll<0, QF>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int q = face_q<Q>(1, -1, k);
+            fout[yf_off<Q>(p, 0, by, k, z, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+        });
+    if (ty == TY - 1 && by < p.nby - 1)
+        unroll<0, QF>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int q = face_q<Q>(1, 1, k);
+            fout[yf_off<Q>(p, 1, by, k, z, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+        });
+    if (z == z0 && c > 0)
+        unroll<0, QF>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int q = face_q<Q>(2, -1, k);
+            fout[zf_off<Q>(p, 0, c, k, y, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+        });
+    if (z == z1 && c < p.nc - 1)
+        unroll<0, QF>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int q = face_q<Q>(2, 1, k);
+            fout[zf_off<Q>(p, 1, c, k, y, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+        });
+}
+
+template <int TX, int TY>
+__device__ __forceinline__ int64_t mom_index(const StepParams &p, int bx, int by, int z, int tid) {
+    return ((((int64_t)z * p.nby + by) * p.nbx + bx) * (TX * TY)) + tid;
+}
+
+template <typename ST, int TX, int TY>
+__device__ __forceinline__ void load_plane(const StepParams &p, int bx, int by, int z, int tid,
+                                           float (&m)[10]) {
+    const int64_t i = mom_index<TX, TY>(p, bx, by, z, tid);
+#pragma unroll
+    for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], i, p.dec[k]);
+}
+
+// The fused step.  PROBE replaces the moment algebra by integer ids so that
+// tests can read back the exact source of every population.
+template <int Q, int TX, int TY, typename ST, bool FO, bool PROBE>
+__global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
+    using G = Geo<TX, TY>;
+    using R = Ring<Q, TX, TY>;
+    constexpr int QF = VS<Q>::QF;
+    __shared__ float lds[VS<Q>::NSLOT * G::P];
+
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const int b = blockIdx.x;
+    const int bx = b % p.nbx;
+    const int by = (b / p.nbx) % p.nby;
+    const int c = b / (p.nbx * p.nby);
+    const int x = bx * TX + tx, y = by * TY + ty;
+    const int z0 = c * p.lz, z1 = z0 + p.lz - 1;
+    const float *__restrict__ fin = p.fin;
+    float *__restrict__ fout = p.fout;
+    const int own = (ty + 1) * G::WP + (tx + 1);
+
+    Job jobs[R::JPT];
+#pragma unroll
+    for (int i = 0; i < R::JPT; i++) jobs[i] = make_job<Q, TX, TY>(tid + i * G::NT, p, bx, by);
+
+    // Up-going populations (cz = +1) of plane z-1 at this column: from the top
+    // face of the chunk below for the first plane (zero below the domain).
+    float up[QF];
+#pragma unroll
+    for (int k = 0; k < QF; k++) up[k] = c > 0 ? fin[zf_off<Q>(p, 1, c - 1, k, y, x)] : 0.f;
+    // Down-going populations (cz = -1) of plane z1+1: bottom face of the chunk above.
+    float dn_end[QF];
+#pragma unroll
+    for (int k = 0; k < QF; k++) dn_end[k] = c < p.nc - 1 ? fin[zf_off<Q>(p, 0, c + 1, k, y, x)] : 0.f;
+
+    float mc[10], mn[10];
+    if constexpr (!PROBE) {
+        load_plane<ST, TX, TY>(p, bx, by, z0, tid, mc);
+        if (p.lz > 1) load_plane<ST, TX, TY>(p, bx, by, z0 + 1, tid, mn);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 10; k++) mc[k] = mn[k] = 0.f;
+    }
+    float ring[R::JPT];
+#pragma unroll
+    for (int i = 0; i < R::JPT; i++) ring[i] = jobs[i].lds >= 0 ? job_load(jobs[i], fin, z0, p.nz) : 0.f;
+
+    for (int z = z0; z <= z1; z++) {
+        // ---- prefetch plane z+2 and the ring of plane z+1
+        float mq[10];
+        if constexpr (!PROBE) {
+            if (z + 2 <= z1) load_plane<ST, TX, TY>(p, bx, by, z + 2, tid, mq);
+        }
+        float ring_n[R::JPT];
+#pragma unroll
+        for (int i = 0; i < R::JPT; i++)
+            ring_n[i] = (jobs[i].lds >= 0 && z + 1 <= z1) ? job_load(jobs[i], fin, z + 1, p.nz) : 0.f;
+
+        // ---- own populations into LDS
+        float g[27];
+        float up_new[QF];
+        {
+            const Rec r = rec_setup(mc);
+            g[0] = pop_of<Q, 0, PROBE>(r, mc, x, y, z, p);
+            unroll<1, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (czq(q) == 0) {
+                    lds[lds_slot(q) * G::P + own] = pop_of<Q, q, PROBE>(r, mc, x, y, z, p);
+                } else if constexpr (czq(q) == 1) {
+                    up_new[face_k<Q>(2, q)] = pop_of<Q, q, PROBE>(r, mc, x, y, z, p);
+                }
+            });
+        }
+        unroll<1, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (czq(q) == 1) {
+                constexpr int k = face_k<Q>(2, q);
+                if constexpr (q == 5)
+                    g[5] = up[k];
+                else
+                    lds[lds_slot(q) * G::P + own] = up[k];
+            }
+        });
+        if (z < z1) {
+            const Rec r = rec_setup(mn);
+            unroll<1, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (czq(q) == -1) {
+                    const float v = pop_of<Q, q, PROBE>(r, mn, x, y, z + 1, p);
+                    if constexpr (q == 6)
+                        g[6] = v;
+                    else
+                        lds[lds_slot(q) * G::P + own] = v;
+                }
+            });
+        } else {
+            unroll<1, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (czq(q) == -1) {
+                    constexpr int k = face_k<Q>(2, q);
+                    if constexpr (q == 6)
+                        g[6] = dn_end[k];
+                    else
+                        lds[lds_slot(q) * G::P + own] = dn_end[k];
+                }
+            });
+        }
+#pragma unroll
+        for (int i = 0; i < R::JPT; i++)
+            if (jobs[i].lds >= 0) lds[jobs[i].lds] = ring[i];
+
+        lds_barrier();
+        unroll<1, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (q != 5 && q != 6)
+                g[q] = lds[lds_slot(q) * G::P + own - cyq(q) * G::WP - cxq(q)];
+        });
+        lds_barrier();
+
+        const int bits = node_bits(x, y, z, p.nx, p.ny, p.nz);
+        if constexpr (PROBE) {
+            const int64_t d = (int64_t)x + (int64_t)p.nx * ((int64_t)y + (int64_t)p.ny * z);
+#pragma unroll
+            for (int q = 0; q < Q; q++) p.probe_src[d * Q + q] = __float_as_uint(g[q]);
+            p.probe_type[d] = (uint8_t)bits;
+            float mz[10];
+#pragma unroll
+            for (int k = 0; k < 10; k++) mz[k] = 0.f;
+            write_faces<Q, TX, TY, true>(p, fout, mz, tx, ty, x, y, z, bx, by, c, z0, z1);
+        } else {
+            const Sums s = pop_sums<Q>(g);
+            float m[10];
+            if (bits & 0x40)
+                boundary_moments<Q>(s, bits, p.omega, p.u_lid, m);
+            else
+                interior_moments(s, m);
+            collide(m, p.t_omega, p.omega_h, p.omega);
+            const int64_t i = mom_index<TX, TY>(p, bx, by, z, tid);
+#pragma unroll
+            for (int k = 0; k < 10; k++) Store<ST>::store(p.mom[k], i, m[k], p.enc[k]);
+            if constexpr (sizeof(ST) < 4) {
+                // neighbours must see the same (rounded) moments the next step reloads
+                float mr[10];
+#pragma unroll
+                for (int k = 0; k < 10; k++) mr[k] = float(ST(m[k] * p.enc[k])) * p.dec[k];
+                write_faces<Q, TX, TY, false>(p, fout, mr, tx, ty, x, y, z, bx, by, c, z0, z1);
+            } else {
+                write_faces<Q, TX, TY, false>(p, fout, m, tx, ty, x, y, z, bx, by, c, z0, z1);
+            }
+            if constexpr (FO) {
+                // kineticEnergy.cuh:71-85 and strainRateTensor.cuh:71-84 on the
+                // stored (scaled) post-collision moments; FP64 running means.
+                const float kk = 0.5f * sqrtf(m[1] * m[1] + m[2] * m[2] + m[3] * m[3]);
+                const float it = 1.0f / p.tau;
+                float S[6];
+                S[0] = 3.f * (m[1] * m[1] - m[4]) / 9.f * it;
+                S[1] = 3.f * (m[1] * m[2] - m[5]) / 18.f * it;
+                S[2] = 3.f * (m[1] * m[3] - m[6]) / 18.f * it;
+                S[3] = 3.f * (m[2] * m[2] - m[7]) / 9.f * it;
+                S[4] = 3.f * (m[2] * m[3] - m[8]) / 18.f * it;
+                S[5] = 3.f * (m[3] * m[3] - m[9]) / 9.f * it;
+                if (p.fo_mask & 1u) p.k[i] = kk;
+                if (p.fo_mask & 4u) {
+#pragma unroll
+                    for (int a = 0; a < 6; a++) p.S[a * p.n + i] = S[a];
+                }
+                if (p.fo_mask & 10u) {
+                    const double inv = 1.0 / double(p.tcount[0] + 1);
+                    if (p.fo_mask & 2u) {
+                        const double o = p.kmean[i];
+                        p.kmean[i] = o + (double(kk) - o) * inv;
+                    }
+                    if (p.fo_mask & 8u) {
+#pragma unroll
+                        for (int a = 0; a < 6; a++) {
+                            const double o = p.smean[a * p.n + i];
+                            p.smean[a * p.n + i] = o + (double(S[a]) - o) * inv;
+                        }
+                    }
+                }
+            }
+        }
+
+        // ---- rotate the plane pipeline
+#pragma unroll
+        for (int k = 0; k < QF; k++) up[k] = up_new[k];
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+            mc[k] = mn[k];
+            if constexpr (!PROBE) mn[k] = mq[k];
+        }
+#pragma unroll
+        for (int i = 0; i < R::JPT; i++) ring[i] = ring_n[i];
+    }
+}
+
+// Face buffers from the current moments (post-collision populations of every
+// face node): the device-side replacement of haloFace::initialise_pop
+// (haloFace.cuh:244-449).  Same grid as the step.
+template <int Q, int TX, int TY, typename ST, bool PROBE>
+__global__ __launch_bounds__(TX *TY) void fill_faces_kernel(const StepParams p) {
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const int b = blockIdx.x;
+    const int bx = b % p.nbx;
+    const int by = (b / p.nbx) % p.nby;
+    const int c = b / (p.nbx * p.nby);
+    const int x = bx * TX + tx, y = by * TY + ty;
+    const int z0 = c * p.lz, z1 = z0 + p.lz - 1;
+    for (int z = z0; z <= z1; z++) {
+        float m[10];
+        if constexpr (!PROBE)
+            load_plane<ST, TX, TY>(p, bx, by, z, tid, m);
+        else
+            for (int k = 0; k < 10; k++) m[k] = 0.f;
+        write_faces<Q, TX, TY, PROBE>(p, p.fout, m, tx, ty, x, y, z, bx, by, c, z0, z1);
+    }
+}
+
+}  // namespace lbm

@@ -1,0 +1,137 @@
+/*
+ * lbm_amd.h -- C ABI of the MI355X moment-representation LBM solver.
+ *
+ * The reference (geoenergiaUDESC/cudaLBM) exposes this path through a CUDA
+ * kernel signature, a CLI and its output files, not through an FFI:
+ *   - kernel   applications/momentBasedD3Q19/momentBasedD3Q19.cuh:88-91
+ *              applications/momentBasedD3Q27/momentBasedD3Q27.cuh:88-91
+ *   - driver   applications/momentBasedD3Q19/momentBasedD3Q19.cu:56-140
+ *   - constants uploaded by host::latticeMesh (src/latticeMesh/latticeMesh.cuh:178-209)
+ * Each entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative lbm_status on error;
+ *     lbm_last_error() returns the message of the last failure on a handle
+ *     (the reference exits the process instead: src/errorHandler.cuh:88-168).
+ *   - Moments are the reference's stored values, in this order:
+ *     rho-1, 3u, 3v, 3w, 4.5(Pxx/rho-cs2), 9Pxy/rho, 9Pxz/rho,
+ *     4.5(Pyy/rho-cs2), 9Pyz/rho, 4.5(Pzz/rho-cs2)
+ *     (src/globalFunctions.cuh:482-503, src/velocitySet/velocitySet.cuh:130-142).
+ *   - Host buffers are owned by the caller; all device memory is owned by the
+ *     handle.  One handle = one HIP stream; calls on a handle must not overlap.
+ *   - No C++ exception crosses this boundary.
+ */
+#ifndef LBM_AMD_H
+#define LBM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lbm_handle lbm_handle;
+
+typedef enum {
+    LBM_OK = 0,
+    LBM_ERR_ARG = -1,      /* bad argument or configuration */
+    LBM_ERR_SIZE = -49,    /* mesh not a multiple of the tile (reference ERR_SIZE) */
+    LBM_ERR_HIP = -2,      /* HIP runtime failure */
+    LBM_ERR_NOMEM = -3,    /* device or host allocation failed */
+    LBM_ERR_STATE = -4     /* call not valid in the handle's current state */
+} lbm_status;
+
+typedef enum { LBM_STORE_FP32 = 0, LBM_STORE_FP16 = 1 } lbm_storage;
+
+/* Host array layouts accepted by lbm_set_moments / lbm_get_moments. */
+typedef enum {
+    LBM_LAYOUT_GLOBAL = 0, /* idx = x + nx*(y + ny*z) */
+    LBM_LAYOUT_TILED8 = 1  /* reference 8^3 block order (src/globalFunctions.cuh:347-370),
+                              the order .LBMBin files carry */
+} lbm_layout;
+
+/* Function-object selection bits (src/functionObjects/functionObjects.cuh:65-70). */
+enum {
+    LBM_FO_K = 1u << 0,     /* k, instantaneous (FP32) */
+    LBM_FO_KMEAN = 1u << 1, /* kMean, running mean (FP64 accumulator) */
+    LBM_FO_S = 1u << 2,     /* S_xx S_xy S_xz S_yy S_yz S_zz, instantaneous (FP32) */
+    LBM_FO_SMEAN = 1u << 3  /* SMean, running means (FP64 accumulators) */
+};
+
+/* Which FO field to read back with lbm_fo_get. */
+typedef enum { LBM_FIELD_K = 0, LBM_FIELD_KMEAN = 1, LBM_FIELD_S = 2, LBM_FIELD_SMEAN = 3 } lbm_fo_field;
+
+typedef struct lbm_config {
+    int velocity_set; /* 19 or 27 */
+    int storage;      /* lbm_storage */
+    int nx, ny, nz;
+    double Re;        /* omega = 1/(0.5 + 3 u_inf (nx-1)/Re), src/latticeMesh/latticeMesh.cuh:178-182 */
+    double u_inf;     /* lid speed: U_North = {u_inf,0,0}, src/latticeMesh/latticeMesh.cuh:183 */
+    int device;       /* HIP device ordinal (reference: -GPU <id>) */
+    void *stream;     /* hipStream_t to run on, or NULL for a handle-owned stream */
+    int tile_x, tile_y; /* workgroup footprint; 0 = choose automatically */
+    int tile_z;         /* planes marched per workgroup; 0 = choose automatically */
+} lbm_config;
+
+/* Replaces host::latticeMesh + device::array + device::halo construction
+ * (momentBasedD3Q19.cu:66-98).  Allocates the moments (initialised to rest)
+ * and both face-buffer sets. */
+int lbm_create(const lbm_config *cfg, lbm_handle **out);
+int lbm_destroy(lbm_handle *h);
+const char *lbm_last_error(const lbm_handle *h);
+/* Message of the last failure of lbm_create (no handle exists then). */
+const char *lbm_create_error(void);
+
+/* Geometry actually chosen (after autotuning) and derived constants. */
+int lbm_get_tiling(const lbm_handle *h, int *tile_x, int *tile_y, int *tile_z);
+int lbm_get_omega(const lbm_handle *h, double *omega, double *tau);
+
+/* Replaces the host::array initial-condition/restart upload (hostArray.cuh:264-296,
+ * memory.cuh:274-282) and haloFace::initialise_pop (haloFace.cuh:244-449): uploads
+ * the 10 moment arrays and rebuilds the face buffers from them on the device. */
+int lbm_set_moments(lbm_handle *h, const float *const m[10], int layout);
+/* Replaces host::toHost(devPtrs) (memory.cuh:157-195). */
+int lbm_get_moments(lbm_handle *h, float *m[10], int layout);
+/* FP64 upload/download variants (values are rounded to the storage type). */
+int lbm_set_moments_f64(lbm_handle *h, const double *const m[10], int layout);
+int lbm_get_moments_f64(lbm_handle *h, double *m[10], int layout);
+
+/* Replaces the per-step kernel launch + blockHalo.swap() + runTimeObjects.calculate()
+ * of the driver loop (momentBasedD3Q19.cu:126-136).  Advances n steps; launches are
+ * recorded once into a hipGraph per (n, face parity) and replayed. Asynchronous. */
+int lbm_step(lbm_handle *h, int n);
+int lbm_synchronize(lbm_handle *h);
+/* Timed variant: hipEvents on the handle's stream around n steps; returns the
+ * elapsed milliseconds (includes only device time of the n steps). */
+int lbm_step_timed(lbm_handle *h, int n, double *ms);
+/* Number of steps taken since creation / the last lbm_set_time. */
+int lbm_get_time(const lbm_handle *h, int64_t *t);
+int lbm_set_time(lbm_handle *h, int64_t t);
+
+/* Replaces objectRegistry (objectRegistry.cuh:68-117): enables fused function
+ * objects computed inside the step from the post-collision moments. Means use
+ * FP64 accumulators and invNewCount = 1/(t+1) of the absolute step t
+ * (kineticEnergy.cuh:249).  Enabling resets the means to zero. */
+int lbm_fo_enable(lbm_handle *h, unsigned mask);
+/* Reads an FO field.  LBM_FIELD_K/KMEAN: n values; S/SMEAN: 6*n values in the
+ * order xx,xy,xz,yy,yz,zz, component-major. */
+int lbm_fo_get(lbm_handle *h, int field, double *out, int layout);
+
+/* Global FP64 reductions (no reference counterpart): kinetic energy
+ * E = sum_nodes 0.5*rho*|u|^2 (physical u) and the velocity residual
+ * r = ||u_t - u_prev||_2 / ||u_t||_2 against the velocity field stored by the
+ * previous call (r = 1 on the first call).  Deterministic (fixed-order). */
+int lbm_reduce(lbm_handle *h, double *kinetic_energy, double *residual);
+
+/* Integer probes of the streaming/halo machinery (test hooks; no reference
+ * counterpart).  src_out[d*Q+q] receives 1 + 32*s + q where s is the global
+ * x-fastest index of the node whose population q arrives at node d, or 0 when
+ * that population is unknown (source outside the domain).  type_out[d] is the
+ * node-type bitmask of src/boundaryConditions/normalVector.cuh:321-334. */
+int lbm_probe_streaming(lbm_handle *h, uint32_t *src_out, uint8_t *type_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
