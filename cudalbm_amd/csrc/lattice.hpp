@@ -1,11 +1,12 @@
 // lattice.hpp -- compile-time velocity sets and pointwise moment algebra shared by
 // every kernel of the solver (gfx950).
 //
-// Populations are carried as deviations from the rest weights, g_q = f_q - w_q,
-// so that the moment sums never cancel against the O(1) rest state in FP32:
-//   rho - 1 = sum g_q,   rho u_a = sum g_q c_a,
-//   Pi_ab = sum g_q c_a c_b + cs2 delta_ab.
-// A population whose source lies outside the domain is carried as g = 0, which
+// Populations are carried as scaled deviations from the rest weights,
+// H_q = 3 D (f_q - w_q), so that the moment sums never cancel against the O(1)
+// rest state in FP32 and every constant is exact:
+//   3 D (rho - 1) = sum H_q,   3 D rho u_a = sum H_q c_a,
+//   3 D (Pi_ab - cs2 delta_ab) = sum H_q c_a c_b.
+// A population whose source lies outside the domain is carried as H = 0, which
 // makes the reference's masked boundary sums (velocitySet.cuh:337-396) plain
 // sums plus a per-node-type constant (the weights of the known populations).
 //
@@ -48,6 +49,7 @@ template <int Q>
 struct VS {
     static constexpr int QF = Q == 19 ? 5 : 9;  // populations crossing one face
     static constexpr int NSLOT = Q - 3;         // LDS slots: every q but 0, 5, 6
+    static constexpr int NU = Q == 19 ? 4 : 8;  // up-going pops that also move in-plane
 };
 
 // LDS slot of a population that moves in-plane (q not in {0,5,6}).
@@ -77,7 +79,21 @@ __host__ __device__ constexpr int face_k(int axis, int q) {
 }
 
 // ---------------------------------------------------------------- constants
-constexpr float kCs2 = 1.0f / 3.0f;
+// Integer weights: w_q = Wint_q / D with D = 36 (D3Q19) or 216 (D3Q27).
+template <int Q>
+__host__ __device__ constexpr int wint(int q) {
+    return Q == 19 ? (absi(cxq(q)) + absi(cyq(q)) + absi(czq(q)) == 0   ? 12
+                      : absi(cxq(q)) + absi(cyq(q)) + absi(czq(q)) == 1 ? 2
+                                                                        : 1)
+                   : (absi(cxq(q)) + absi(cyq(q)) + absi(czq(q)) == 0   ? 64
+                      : absi(cxq(q)) + absi(cyq(q)) + absi(czq(q)) == 1 ? 16
+                      : absi(cxq(q)) + absi(cyq(q)) + absi(czq(q)) == 2 ? 4
+                                                                        : 1);
+}
+template <int Q>
+__host__ __device__ constexpr float d3() {  // 3 D
+    return Q == 19 ? 108.f : 648.f;
+}
 
 // Compile-time loop.
 template <int B, int E, typename F>
@@ -88,27 +104,33 @@ __device__ __forceinline__ void unroll(F &&f) {
     }
 }
 
+// Populations travel as H_q = 3 D (f_q - w_q).  Every constant of the
+// reconstruct / moment round trip is then an exact small integer or dyadic, so
+// FP32 rounding is unbiased: roundings of 1/18, 1/36 or 1/3 would otherwise
+// scale the state by the same factor on every step.
+//
 // Moment vector m: [0] rho-1, [1..3] 3u, [4] mxx, [5] mxy, [6] mxz, [7] myy,
 // [8] myz, [9] mzz (scaled, as stored: velocitySet.cuh:130-142).
 struct Rec {
-    float rpw_base;  // rho - 1
-    float rho;       // rho
-    float A;         // -cs2 (mxx + myy + mzz)
+    float r3;   // 3 (rho - 1)
+    float rho;  // rho
+    float A;    // -(mxx + myy + mzz)
 };
 
 __device__ __forceinline__ Rec rec_setup(const float (&m)[10]) {
     Rec r;
-    r.rpw_base = m[0];
+    r.r3 = 3.0f * m[0];
     r.rho = 1.0f + m[0];
-    r.A = -kCs2 * (m[4] + m[7] + m[9]);
+    r.A = -(m[4] + m[7] + m[9]);
     return r;
 }
 
-// g_q = f_q - w_q with f_q = rho w_q [1 + B_q] (D3Q19.cuh:346-374): g_q = w_q (rho' + rho B_q).
+// f_q = rho w_q [1 - cs2(mxx+myy+mzz) + c.m + ...] (D3Q19.cuh:346-374), so
+// H_q = Wint_q (3 rho' + rho T_q), T_q = 3 (c_a m_a + c_a^2 m_aa + c_a c_b m_ab) - (mxx+myy+mzz).
 template <int Q, int q>
 __device__ __forceinline__ float rec_dev(const Rec &r, const float (&m)[10]) {
     constexpr int x = cxq(q), y = cyq(q), z = czq(q);
-    float B = r.A;
+    float B = 0.f;
     if constexpr (x != 0) B += float(x) * m[1];
     if constexpr (y != 0) B += float(y) * m[2];
     if constexpr (z != 0) B += float(z) * m[3];
@@ -118,11 +140,11 @@ __device__ __forceinline__ float rec_dev(const Rec &r, const float (&m)[10]) {
     if constexpr (x * y != 0) B += float(x * y) * m[5];
     if constexpr (x * z != 0) B += float(x * z) * m[6];
     if constexpr (y * z != 0) B += float(y * z) * m[8];
-    constexpr float w = float(wq<Q>(q));
-    return w * fmaf(r.rho, B, r.rpw_base);
+    const float T = fmaf(3.0f, B, r.A);
+    return float(wint<Q>(q)) * fmaf(r.rho, T, r.r3);
 }
 
-// Raw second-order sums of the deviation populations.
+// Raw second-order sums of H.
 struct Sums {
     float s0, sx, sy, sz, sxx, sxy, sxz, syy, syz, szz;
 };
@@ -148,130 +170,121 @@ __device__ __forceinline__ Sums pop_sums(const float (&g)[27]) {
     return s;
 }
 
-// Interior moments (velocitySet.cuh:398-417) followed by scale (velocitySet.cuh:130-142).
+// Interior moments (velocitySet.cuh:398-417) followed by scale (velocitySet.cuh:130-142):
+// rho' = S0/3D, R = 3D rho = 3D + S0, 3u = 3 Sx/R, mxx = (4.5 Sxx - 1.5 S0)/R, mxy = 9 Sxy/R.
+template <int Q>
 __device__ __forceinline__ void interior_moments(const Sums &s, float (&m)[10]) {
-    const float rho = 1.0f + s.s0;
-    const float inv = 1.0f / rho;
-    m[0] = s.s0;
+    const float inv = 1.0f / (d3<Q>() + s.s0);
+    m[0] = s.s0 / d3<Q>();
     m[1] = 3.0f * s.sx * inv;
     m[2] = 3.0f * s.sy * inv;
     m[3] = 3.0f * s.sz * inv;
-    m[4] = 4.5f * (fmaf(-kCs2, s.s0, s.sxx) * inv);
+    m[4] = fmaf(4.5f, s.sxx, -1.5f * s.s0) * inv;
     m[5] = 9.0f * s.sxy * inv;
     m[6] = 9.0f * s.sxz * inv;
-    m[7] = 4.5f * (fmaf(-kCs2, s.s0, s.syy) * inv);
+    m[7] = fmaf(4.5f, s.syy, -1.5f * s.s0) * inv;
     m[8] = 9.0f * s.syz * inv;
-    m[9] = 4.5f * (fmaf(-kCs2, s.s0, s.szz) * inv);
+    m[9] = fmaf(4.5f, s.szz, -1.5f * s.s0) * inv;
 }
 
-// Boundary moments: boundaryConditions.cuh:99-554, in scaled form.  `bits` is the
-// normalVector bitmask (W=1, E=2, S=4, N=8, B=16, F=32).
+// Boundary moments: boundaryConditions.cuh:99-554 rewritten in deviation form.
+// With rho_I = W + rI and (m_ab,I rho_I) = A + a_ab, where W and A are the known
+// weight sums of the node type (exact rationals) and rI, a_ab the known sums of
+// g = f - w, each closure rho = (alpha rho_I + beta a)/gamma becomes
+// rho - 1 = (alpha rI + beta a + c0)/gamma with c0 = alpha W + beta A - gamma,
+// which is identically zero for every static wall, edge and corner (their
+// closures keep the rest state), so it is dropped there and kept in closed form
+// for the lid.  `bits` is the normalVector bitmask (W=1, E=2, S=4, N=8, B=16, F=32).
 template <int Q>
-__device__ __forceinline__ void boundary_moments(const Sums &s, int bits, float omega, float u_lid,
+__device__ __forceinline__ void boundary_moments(const Sums &s, int bits, float omega, float U,
                                                  float (&m)[10]) {
-    // Known-population weight sums for this node type (incomingSwitch,
-    // velocitySet.cuh:204-220): rho_I = W + sum g, Sab_I = Wab + sum g c_a c_b.
-    float W = 0.f, Wxy = 0.f, Wxz = 0.f, Wyz = 0.f;
-    unroll<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int x = cxq(q), y = cyq(q), z = czq(q);
-        bool unknown = false;
-        if constexpr (x > 0) unknown |= (bits & 1) != 0;
-        if constexpr (x < 0) unknown |= (bits & 2) != 0;
-        if constexpr (y > 0) unknown |= (bits & 4) != 0;
-        if constexpr (y < 0) unknown |= (bits & 8) != 0;
-        if constexpr (z > 0) unknown |= (bits & 16) != 0;
-        if constexpr (z < 0) unknown |= (bits & 32) != 0;
-        constexpr float w = float(wq<Q>(q));
-        const float k = unknown ? 0.f : w;
-        W += k;
-        if constexpr (x * y != 0) Wxy += float(x * y) * k;
-        if constexpr (x * z != 0) Wxz += float(x * z) * k;
-        if constexpr (y * z != 0) Wyz += float(y * z) * k;
-    });
-    const float rho_I = W + s.s0;
-    const float inv_I = 1.0f / rho_I;
-    const float mxy_I = (Wxy + s.sxy) * inv_I;
-    const float mxz_I = (Wxz + s.sxz) * inv_I;
-    const float myz_I = (Wyz + s.syz) * inv_I;
-
+    const float rI = s.s0 / d3<Q>();
+    const float axy = s.sxy / d3<Q>(), axz = s.sxz / d3<Q>(), ayz = s.syz / d3<Q>();
+    const float w = omega, tw = 1.0f - omega;
     const bool north = (bits & 8) != 0;
     const int nb = __popc(bits & 0x3F);
-    const float ux = north ? u_lid / float(nb) : 0.f;  // U<X> (boundaryConditions.cuh:543-554)
-    const float w = omega, U = u_lid;
-    float rho = 1.f, mxy = 0.f, mxz = 0.f, myz = 0.f;
-    auto edge = [&](float sign, float mI, float &mab) {
-        // rho = 36 (rho_I - mI rho_I + mI rho_I w)/(24 + w) for sign +1, and
-        // -36(-rho_I - mI rho_I + mI rho_I w)/(24 + w) for sign -1.
-        const float a = mI * rho_I;
-        rho = (sign > 0.f) ? 36.f * (rho_I - a + a * w) / (24.f + w)
-                           : -36.f * (-rho_I - a + a * w) / (24.f + w);
-        mab = (36.f * a - sign * rho) / (9.f * rho);
+    const float ux = north ? U / float(nb) : 0.f;  // U<X> (boundaryConditions.cuh:543-554)
+    float rp = 0.f, mxy = 0.f, mxz = 0.f, myz = 0.f;
+    // static edge: sign +1 is rho = 36(rho_I - a + a w)/(24 + w), mab = (36a - rho)/(9 rho);
+    // sign -1 is rho = -36(-rho_I - a + a w)/(24 + w), mab = (36a + rho)/(9 rho)
+    auto edge = [&](float sign, float a, float &mab) {
+        rp = 36.f * (rI - sign * tw * a) / (24.f + w);
+        mab = (36.f * a - sign * rp) / (9.f * (1.f + rp));
     };
     switch (bits & 0x3F) {
-    case 21: case 37: case 22: case 38:  // SWB SWF SEB SEF
-        rho = (Q == 19) ? 12.f * rho_I / 7.f : 216.f * rho_I / 125.f;
+    case 21: case 37: case 22: case 38:  // SWB SWF SEB SEF: rho = 12 rho_I/7 (216/125 for D3Q27)
+        rp = (Q == 19) ? 12.f * rI / 7.f : 216.f * rI / 125.f;
         break;
-    case 5: edge(+1.f, mxy_I, mxy); break;   // SW
-    case 6: edge(-1.f, mxy_I, mxy); break;   // SE
-    case 17: edge(+1.f, mxz_I, mxz); break;  // WB
-    case 33: edge(-1.f, mxz_I, mxz); break;  // WF
-    case 18: edge(-1.f, mxz_I, mxz); break;  // EB
-    case 34: edge(+1.f, mxz_I, mxz); break;  // EF
-    case 20: edge(+1.f, myz_I, myz); break;  // SB
-    case 36: edge(-1.f, myz_I, myz); break;  // SF
-    case 1: case 2:  // W, E
-        rho = 6.f * rho_I / 5.f;
-        mxy = 2.f * mxy_I * rho_I / rho;
-        mxz = 2.f * mxz_I * rho_I / rho;
-        break;
-    case 4:  // S
-        rho = 6.f * rho_I / 5.f;
-        mxy = 2.f * mxy_I * rho_I / rho;
-        myz = 2.f * myz_I * rho_I / rho;
-        break;
-    case 16: case 32:  // B, F
-        rho = 6.f * rho_I / 5.f;
-        mxz = 2.f * mxz_I * rho_I / rho;
-        myz = 2.f * myz_I * rho_I / rho;
-        break;
-    case 8:  // N (moving lid)
-        rho = 6.f * rho_I / 5.f;
-        mxy = (6.f * mxy_I * rho_I - U * rho) / (3.f * rho);
-        myz = 2.f * myz_I * rho_I / rho;
-        break;
-    case 25: case 41:  // NWB NWF
-        rho = -24.f * rho_I / (-14.f - 8.f * U + 9.f * U * U);
-        break;
-    case 26: case 42:  // NEB NEF
-        rho = -24.f * rho_I / (-14.f + 8.f * U + 9.f * U * U);
-        break;
-    case 24: {  // NB
-        const float a = myz_I * rho_I;
-        rho = 72.f * (-rho_I - a + a * w) / (-48.f - 2.f * w + 3.f * U * U * w);
-        myz = (72.f * a + 2.f * rho - 3.f * U * U * rho) / (18.f * rho);
+    case 5: edge(+1.f, axy, mxy); break;   // SW
+    case 6: edge(-1.f, axy, mxy); break;   // SE
+    case 17: edge(+1.f, axz, mxz); break;  // WB
+    case 33: edge(-1.f, axz, mxz); break;  // WF
+    case 18: edge(-1.f, axz, mxz); break;  // EB
+    case 34: edge(+1.f, axz, mxz); break;  // EF
+    case 20: edge(+1.f, ayz, myz); break;  // SB
+    case 36: edge(-1.f, ayz, myz); break;  // SF
+    case 1: case 2: {  // W, E: rho = 6 rho_I/5, mab = 2 a/rho
+        rp = 6.f * rI / 5.f;
+        const float rho = 1.f + rp;
+        mxy = 2.f * axy / rho;
+        mxz = 2.f * axz / rho;
     } break;
-    case 40: {  // NF
-        const float a = myz_I * rho_I;
-        rho = -72.f * (rho_I - a + a * w) / (-48.f - 2.f * w + 3.f * U * U * w);
-        myz = (72.f * a - 2.f * rho + 3.f * U * U * rho) / (18.f * rho);
+    case 4: {  // S
+        rp = 6.f * rI / 5.f;
+        const float rho = 1.f + rp;
+        mxy = 2.f * axy / rho;
+        myz = 2.f * ayz / rho;
     } break;
-    case 10: {  // NE
-        const float a = mxy_I * rho_I;
-        rho = 36.f * (rho_I - a + a * w) /
-              (24.f - 18.f * U - 18.f * U * U + w + 3.f * U * w + 3.f * U * U * w);
-        mxy = (36.f * a - rho - 3.f * U * rho - 3.f * U * U * rho) / (9.f * rho);
+    case 16: case 32: {  // B, F
+        rp = 6.f * rI / 5.f;
+        const float rho = 1.f + rp;
+        mxz = 2.f * axz / rho;
+        myz = 2.f * ayz / rho;
     } break;
-    case 9: {  // NW
-        const float a = mxy_I * rho_I;
-        rho = -36.f * (-rho_I - a + a * w) /
-              (24.f + 18.f * U - 18.f * U * U + w - 3.f * U * w + 3.f * U * U * w);
-        mxy = (36.f * a + rho - 3.f * U * rho + 3.f * U * U * rho) / (9.f * rho);
+    case 8: {  // N (moving lid): mxy = (6a - U rho)/(3 rho), myz = 2a/rho
+        rp = 6.f * rI / 5.f;
+        const float rho = 1.f + rp;
+        mxy = (6.f * axy - U * rho) / (3.f * rho);
+        myz = 2.f * ayz / rho;
+    } break;
+    case 25: case 41: {  // NWB NWF: rho = 24 rho_I/(14 + 8U - 9U^2); W = 7/12 (125/216 for D3Q27)
+        const float c0 = (Q == 19) ? 0.f : -1.f / 9.f;
+        rp = (24.f * rI + c0 - 8.f * U + 9.f * U * U) / (14.f + 8.f * U - 9.f * U * U);
+    } break;
+    case 26: case 42: {  // NEB NEF: rho = 24 rho_I/(14 - 8U - 9U^2)
+        const float c0 = (Q == 19) ? 0.f : -1.f / 9.f;
+        rp = (24.f * rI + c0 + 8.f * U + 9.f * U * U) / (14.f - 8.f * U - 9.f * U * U);
+    } break;
+    case 24: {  // NB: rho = 72(-rho_I - a + a w)/(-48 - 2w + 3U^2 w)
+        const float g = -48.f - 2.f * w + 3.f * U * U * w;
+        rp = (-72.f * rI - 72.f * tw * ayz - 3.f * U * U * w) / g;
+        const float rho = 1.f + rp;
+        myz = (72.f * ayz + 2.f * rp - 3.f * U * U * rho) / (18.f * rho);
+    } break;
+    case 40: {  // NF: rho = -72(rho_I - a + a w)/(-48 - 2w + 3U^2 w)
+        const float g = -48.f - 2.f * w + 3.f * U * U * w;
+        rp = (-72.f * rI + 72.f * tw * ayz - 3.f * U * U * w) / g;
+        const float rho = 1.f + rp;
+        myz = (72.f * ayz - 2.f * rp + 3.f * U * U * rho) / (18.f * rho);
+    } break;
+    case 10: {  // NE: rho = 36(rho_I - a + a w)/g
+        const float g = 24.f - 18.f * U - 18.f * U * U + w + 3.f * U * w + 3.f * U * U * w;
+        const float c0 = 18.f * U + 18.f * U * U - 3.f * U * w - 3.f * U * U * w;  // 24 + w - g
+        rp = (36.f * rI - 36.f * tw * axy + c0) / g;
+        const float rho = 1.f + rp;
+        mxy = (36.f * axy - rp - 3.f * U * rho - 3.f * U * U * rho) / (9.f * rho);
+    } break;
+    case 9: {  // NW: rho = -36(-rho_I - a + a w)/g
+        const float g = 24.f + 18.f * U - 18.f * U * U + w - 3.f * U * w + 3.f * U * U * w;
+        const float c0 = -18.f * U + 18.f * U * U + 3.f * U * w - 3.f * U * U * w;  // 24 + w - g
+        rp = (36.f * rI + 36.f * tw * axy + c0) / g;
+        const float rho = 1.f + rp;
+        mxy = (36.f * axy + rp - 3.f * U * rho + 3.f * U * U * rho) / (9.f * rho);
     } break;
     default: break;
     }
     // unscaled -> scaled (velocitySet.cuh:130-142); m_xx = u_x^2, m_yy = m_zz = 0
-    m[0] = rho - 1.0f;
+    m[0] = rp;
     m[1] = 3.f * ux;
     m[2] = 0.f;
     m[3] = 0.f;

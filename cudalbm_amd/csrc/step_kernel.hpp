@@ -42,7 +42,7 @@ struct StepParams {
     void *mom[10];         // stored moments, column-tiled internal layout, in place
     const float *fin;      // face buffers read (previous step): X | Y | Z
     float *fout;           // face buffers written (this step)
-    int64_t yoff, zoff;    // offsets of the Y and Z face blocks inside one face set
+    int yoff, zoff;        // offsets of the Y and Z face blocks inside one face set
     float enc[10], dec[10];  // FP16 storage codec: stored16 = m*enc, m = stored16*dec
     // fused function objects
     unsigned fo_mask;
@@ -61,19 +61,19 @@ template <typename ST>
 struct Store;
 template <>
 struct Store<float> {
-    __device__ static __forceinline__ float load(const void *p, int64_t i, float) {
+    __device__ static __forceinline__ float load(const void *p, int i, float) {
         return __builtin_nontemporal_load(static_cast<const float *>(p) + i);
     }
-    __device__ static __forceinline__ void store(void *p, int64_t i, float v, float) {
+    __device__ static __forceinline__ void store(void *p, int i, float v, float) {
         __builtin_nontemporal_store(v, static_cast<float *>(p) + i);
     }
 };
 template <>
 struct Store<_Float16> {
-    __device__ static __forceinline__ float load(const void *p, int64_t i, float dec) {
+    __device__ static __forceinline__ float load(const void *p, int i, float dec) {
         return float(static_cast<const _Float16 *>(p)[i]) * dec;
     }
-    __device__ static __forceinline__ void store(void *p, int64_t i, float v, float enc) {
+    __device__ static __forceinline__ void store(void *p, int i, float v, float enc) {
         static_cast<_Float16 *>(p)[i] = _Float16(v * enc);
     }
 };
@@ -114,95 +114,103 @@ struct Ring {
     static constexpr int JPT = (NJ + TX * TY - 1) / (TX * TY);
 };
 
+// Index of an up-going, in-plane-moving population among its kind.
+template <int Q>
+__host__ __device__ constexpr int u_index(int q) {
+    int n = 0;
+    for (int p = 1; p < q; p++)
+        if (czq(p) == 1 && (cxq(p) != 0 || cyq(p) != 0)) n++;
+    return n;
+}
+// LDS base of the slot holding up-going population q in bank 0 or 1.
+template <int Q, int TX, int TY>
+__host__ __device__ constexpr int u_addr(int q, int bank) {
+    return bank ? (VS<Q>::NSLOT + u_index<Q>(q)) * Geo<TX, TY>::P : lds_slot(q) * Geo<TX, TY>::P;
+}
 struct Job {
-    int lds;      // LDS word offset, or -1 when the thread has no job
-    int64_t gb;   // face-set offset without the plane term, or -1 when outside the domain
-    int zs;       // plane stride inside the face block
-    int dz;       // source plane offset: -cz
+    int lds;  // LDS word offset (bank 0), or -1 when the thread has no job
+    int lds1; // LDS word offset when up-going pops use bank 1
+    int gb;   // face-set offset without the plane term, or -1 when outside the domain
+    int zs;   // plane stride inside the face block
+    int dz;   // source plane offset: -cz
 };
 
+// Ring job j of this workgroup: (side, face slot k, position) decoded against the
+// compile-time segment table, so the population index is a constant per branch.
 template <int Q, int TX, int TY>
 __device__ __forceinline__ Job make_job(int j, const StepParams &p, int bx, int by) {
-    using R = Ring<Q, TX, TY>;
     using G = Geo<TX, TY>;
     constexpr int QF = VS<Q>::QF;
-    Job job{-1, -1, 0, 0};
-    if (j >= R::NJ) return job;
-    int side = 0, k = 0, pos = j;
-    // locate (side, k, pos)
-    for (int s = 0; s < 4; s++) {
-        bool found = false;
-        for (int kk = 0; kk < QF; kk++) {
-            const int c = R::side_count(s, kk);
-            if (pos < c) {
-                side = s;
-                k = kk;
-                found = true;
-                break;
+    Job job{-1, -1, -1, 0, 0};
+    int start = 0;
+    unroll<0, 4>([&](auto sc) {
+        constexpr int side = decltype(sc)::value;
+        unroll<0, QF>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int axis = side < 2 ? 0 : 1;
+            constexpr int sign = (side == 0 || side == 2) ? 1 : -1;
+            constexpr int q = face_q<Q>(axis, sign, k);
+            constexpr int cnt = Ring<Q, TX, TY>::side_count(side, k);
+            const int pos = j - start;
+            if (pos >= 0 && pos < cnt) {
+                int rx, ry;
+                if constexpr (side < 2) {
+                    rx = side == 0 ? -1 : TX;
+                    ry = pos - cyq(q);  // ry in [-cy, TY-1-cy]
+                } else {
+                    ry = side == 2 ? -1 : TY;
+                    rx = pos + (cxq(q) < 0 ? 1 : 0);  // rx in [max(0,-cx), min(TX-1, TX-1-cx)]
+                }
+                const int cell = (ry + 1) * G::WP + (rx + 1);
+                job.lds = lds_slot(q) * G::P + cell;
+                job.lds1 = czq(q) == 1 ? u_addr<Q, TX, TY>(q, 1) + cell : job.lds;
+                job.dz = -czq(q);
+                const int gx = bx * TX + rx, gy = by * TY + ry;
+                const bool inside = gx >= 0 && gx < p.nx && gy >= 0 && gy < p.ny;
+                if constexpr (side < 2) {
+                    const int col = side == 0 ? bx - 1 : bx + 1;
+                    constexpr int s_face = side == 0 ? 1 : 0;  // east (1) / west (0) faces
+                    job.zs = p.ny;
+                    job.gb = inside ? (((s_face * p.nbx + col) * QF + k) * p.nz) * p.ny + gy : -1;
+                } else {
+                    const int row = side == 2 ? by - 1 : by + 1;
+                    constexpr int s_face = side == 2 ? 1 : 0;
+                    job.zs = p.nx;
+                    job.gb = inside ? p.yoff + (((s_face * p.nby + row) * QF + k) * p.nz) * p.nx + gx : -1;
+                }
             }
-            pos -= c;
-        }
-        if (found) break;
-    }
-    const int axis = side < 2 ? 0 : 1;
-    const int sign = (side == 0 || side == 2) ? 1 : -1;
-    // population index, by a runtime scan of the compile-time table
-    int q = 0;
-    for (int kk = 0; kk < QF; kk++)
-        if (kk == k) q = face_q<Q>(axis, sign, kk);
-    // the population table is small; recompute its components at run time
-    int qcx = 0, qcy = 0, qcz = 0;
-    for (int qq = 0; qq < Q; qq++)
-        if (qq == q) {
-            qcx = kC[qq][0];
-            qcy = kC[qq][1];
-            qcz = kC[qq][2];
-        }
-    int rx, ry;
-    if (side < 2) {
-        rx = side == 0 ? -1 : TX;
-        ry = pos - qcy;  // ry in [-cy, TY-1-cy]
-    } else {
-        ry = side == 2 ? -1 : TY;
-        rx = pos + (qcx < 0 ? 1 : 0);  // rx in [max(0,-cx), min(TX-1, TX-1-cx)]
-    }
-    const int slot = q <= 4 ? q - 1 : q - 3;
-    job.lds = slot * G::P + (ry + 1) * G::WP + (rx + 1);
-    job.dz = -qcz;
-    const int gx = bx * TX + rx, gy = by * TY + ry;
-    const bool inside = gx >= 0 && gx < p.nx && gy >= 0 && gy < p.ny;
-    if (side < 2) {
-        const int col = side == 0 ? bx - 1 : bx + 1;
-        const int s_face = side == 0 ? 1 : 0;  // read the east (1) / west (0) faces
-        job.zs = p.ny;
-        job.gb = inside ? ((((int64_t)s_face * p.nbx + col) * QF + k) * p.nz) * p.ny + gy : -1;
-    } else {
-        const int row = side == 2 ? by - 1 : by + 1;
-        const int s_face = side == 2 ? 1 : 0;
-        job.zs = p.nx;
-        job.gb = inside ? p.yoff + ((((int64_t)s_face * p.nby + row) * QF + k) * p.nz) * p.nx + gx : -1;
-    }
+            start += cnt;
+        });
+    });
     return job;
 }
 
-__device__ __forceinline__ float job_load(const Job &j, const float *__restrict__ fin, int zdest, int nz) {
+// Raw face word for destination plane zdest (address clamped into the buffer so
+// the load is unconditional); job_value() applies the validity afterwards.
+__device__ __forceinline__ float job_raw(const Job &j, const float *__restrict__ fin, int zdest, int nz) {
+    int zs = zdest + j.dz;
+    zs = zs < 0 ? 0 : (zs >= nz ? nz - 1 : zs);
+    const int off = j.gb < 0 ? 0 : j.gb + zs * j.zs;
+    return fin[off];
+}
+__device__ __forceinline__ float job_value(const Job &j, float raw, int zdest, int nz) {
     const int zs = zdest + j.dz;
-    if (j.gb < 0 || zs < 0 || zs >= nz) return 0.f;
-    return fin[j.gb + (int64_t)zs * j.zs];
+    return (j.gb >= 0 && zs >= 0 && zs < nz) ? raw : 0.f;
 }
 
 // Offsets of one population inside the face blocks.
+// All offsets are 32-bit: the host rejects meshes whose arrays exceed 2^31 words.
 template <int Q>
-__device__ __forceinline__ int64_t xf_off(const StepParams &p, int s, int bx, int k, int z, int y) {
-    return ((((int64_t)s * p.nbx + bx) * VS<Q>::QF + k) * p.nz + z) * p.ny + y;
+__device__ __forceinline__ int xf_off(const StepParams &p, int s, int bx, int k, int z, int y) {
+    return (((s * p.nbx + bx) * VS<Q>::QF + k) * p.nz + z) * p.ny + y;
 }
 template <int Q>
-__device__ __forceinline__ int64_t yf_off(const StepParams &p, int s, int by, int k, int z, int x) {
-    return p.yoff + ((((int64_t)s * p.nby + by) * VS<Q>::QF + k) * p.nz + z) * p.nx + x;
+__device__ __forceinline__ int yf_off(const StepParams &p, int s, int by, int k, int z, int x) {
+    return p.yoff + (((s * p.nby + by) * VS<Q>::QF + k) * p.nz + z) * p.nx + x;
 }
 template <int Q>
-__device__ __forceinline__ int64_t zf_off(const StepParams &p, int s, int c, int k, int y, int x) {
-    return p.zoff + ((((int64_t)s * p.nc + c) * VS<Q>::QF + k) * p.ny + y) * p.nx + x;
+__device__ __forceinline__ int zf_off(const StepParams &p, int s, int c, int k, int y, int x) {
+    return p.zoff + (((s * p.nc + c) * VS<Q>::QF + k) * p.ny + y) * p.nx + x;
 }
 
 // Population q of node (x,y,z) rebuilt from its moments (or its probe id).
@@ -261,26 +269,35 @@ __device__ __forceinline__ void write_faces(const StepParams &p, float *__restri
 }
 
 template <int TX, int TY>
-__device__ __forceinline__ int64_t mom_index(const StepParams &p, int bx, int by, int z, int tid) {
-    return ((((int64_t)z * p.nby + by) * p.nbx + bx) * (TX * TY)) + tid;
+__device__ __forceinline__ int mom_index(const StepParams &p, int bx, int by, int z, int tid) {
+    return (((z * p.nby + by) * p.nbx + bx) * (TX * TY)) + tid;
 }
 
 template <typename ST, int TX, int TY>
 __device__ __forceinline__ void load_plane(const StepParams &p, int bx, int by, int z, int tid,
                                            float (&m)[10]) {
-    const int64_t i = mom_index<TX, TY>(p, bx, by, z, tid);
+    const int i = mom_index<TX, TY>(p, bx, by, z, tid);
 #pragma unroll
     for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], i, p.dec[k]);
 }
 
 // The fused step.  PROBE replaces the moment algebra by integer ids so that
 // tests can read back the exact source of every population.
+//
+// LDS: NS population slots of (TX+2)x(TY+2) (in-plane and down-going pops of
+// this plane, up-going pops of the previous plane in bank 0), NU more slots for
+// the second bank of up-going pops (the bank alternates with the plane parity so
+// plane z can stage its own up-going pops while plane z-1's are being read), and
+// QF*NT words holding the down-going pops of the plane above the chunk.
 template <int Q, int TX, int TY, typename ST, bool FO, bool PROBE>
 __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
     using G = Geo<TX, TY>;
     using R = Ring<Q, TX, TY>;
     constexpr int QF = VS<Q>::QF;
-    __shared__ float lds[VS<Q>::NSLOT * G::P];
+    constexpr int NS = VS<Q>::NSLOT;
+    constexpr int NU = VS<Q>::NU;
+    __shared__ float lds[(NS + NU) * G::P + QF * G::NT];
+    float *const dnst = lds + (NS + NU) * G::P;
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -298,42 +315,52 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
 #pragma unroll
     for (int i = 0; i < R::JPT; i++) jobs[i] = make_job<Q, TX, TY>(tid + i * G::NT, p, bx, by);
 
-    // Up-going populations (cz = +1) of plane z-1 at this column: from the top
-    // face of the chunk below for the first plane (zero below the domain).
-    float up[QF];
+    // Up-going populations of plane z0-1 at this column (top face of the chunk
+    // below; zero under the domain) into the bank plane z0 reads.
+    float p5;
+    {
+        const int bank = (z0 - 1) & 1;
+        unroll<1, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (czq(q) == 1) {
+                constexpr int k = face_k<Q>(2, q);
+                const float v = c > 0 ? fin[zf_off<Q>(p, 1, c - 1, k, y, x)] : 0.f;
+                if constexpr (q == 5)
+                    p5 = v;
+                else
+                    lds[u_addr<Q, TX, TY>(q, bank) + own] = v;
+            }
+        });
+    }
+    // Down-going populations of plane z1+1 (bottom face of the chunk above).
 #pragma unroll
-    for (int k = 0; k < QF; k++) up[k] = c > 0 ? fin[zf_off<Q>(p, 1, c - 1, k, y, x)] : 0.f;
-    // Down-going populations (cz = -1) of plane z1+1: bottom face of the chunk above.
-    float dn_end[QF];
-#pragma unroll
-    for (int k = 0; k < QF; k++) dn_end[k] = c < p.nc - 1 ? fin[zf_off<Q>(p, 0, c + 1, k, y, x)] : 0.f;
+    for (int k = 0; k < QF; k++) dnst[k * G::NT + tid] = c < p.nc - 1 ? fin[zf_off<Q>(p, 0, c + 1, k, y, x)] : 0.f;
 
     float mc[10], mn[10];
     if constexpr (!PROBE) {
         load_plane<ST, TX, TY>(p, bx, by, z0, tid, mc);
-        if (p.lz > 1) load_plane<ST, TX, TY>(p, bx, by, z0 + 1, tid, mn);
+        load_plane<ST, TX, TY>(p, bx, by, z0 + 1 <= z1 ? z0 + 1 : z1, tid, mn);
     } else {
 #pragma unroll
         for (int k = 0; k < 10; k++) mc[k] = mn[k] = 0.f;
     }
     float ring[R::JPT];
 #pragma unroll
-    for (int i = 0; i < R::JPT; i++) ring[i] = jobs[i].lds >= 0 ? job_load(jobs[i], fin, z0, p.nz) : 0.f;
+    for (int i = 0; i < R::JPT; i++) ring[i] = job_raw(jobs[i], fin, z0, p.nz);
 
     for (int z = z0; z <= z1; z++) {
-        // ---- prefetch plane z+2 and the ring of plane z+1
+        // ---- prefetch plane z+2 and the ring words of plane z+1 (unconditional
+        //      loads of clamped addresses; unused values are discarded)
         float mq[10];
-        if constexpr (!PROBE) {
-            if (z + 2 <= z1) load_plane<ST, TX, TY>(p, bx, by, z + 2, tid, mq);
-        }
+        if constexpr (!PROBE) load_plane<ST, TX, TY>(p, bx, by, z + 2 <= z1 ? z + 2 : z1, tid, mq);
         float ring_n[R::JPT];
 #pragma unroll
-        for (int i = 0; i < R::JPT; i++)
-            ring_n[i] = (jobs[i].lds >= 0 && z + 1 <= z1) ? job_load(jobs[i], fin, z + 1, p.nz) : 0.f;
+        for (int i = 0; i < R::JPT; i++) ring_n[i] = job_raw(jobs[i], fin, z + 1, p.nz);
 
+        const int bank_rd = (z - 1) & 1, bank_wr = z & 1;
         // ---- own populations into LDS
         float g[27];
-        float up_new[QF];
+        float p5_next;
         {
             const Rec r = rec_setup(mc);
             g[0] = pop_of<Q, 0, PROBE>(r, mc, x, y, z, p);
@@ -342,20 +369,15 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
                 if constexpr (czq(q) == 0) {
                     lds[lds_slot(q) * G::P + own] = pop_of<Q, q, PROBE>(r, mc, x, y, z, p);
                 } else if constexpr (czq(q) == 1) {
-                    up_new[face_k<Q>(2, q)] = pop_of<Q, q, PROBE>(r, mc, x, y, z, p);
+                    const float v = pop_of<Q, q, PROBE>(r, mc, x, y, z, p);
+                    if constexpr (q == 5)
+                        p5_next = v;
+                    else
+                        lds[u_addr<Q, TX, TY>(q, bank_wr) + own] = v;
                 }
             });
         }
-        unroll<1, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            if constexpr (czq(q) == 1) {
-                constexpr int k = face_k<Q>(2, q);
-                if constexpr (q == 5)
-                    g[5] = up[k];
-                else
-                    lds[lds_slot(q) * G::P + own] = up[k];
-            }
-        });
+        g[5] = p5;
         if (z < z1) {
             const Rec r = rec_setup(mn);
             unroll<1, Q>([&](auto qc) {
@@ -372,31 +394,35 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
             unroll<1, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 if constexpr (czq(q) == -1) {
-                    constexpr int k = face_k<Q>(2, q);
+                    const float v = dnst[face_k<Q>(2, q) * G::NT + tid];
                     if constexpr (q == 6)
-                        g[6] = dn_end[k];
+                        g[6] = v;
                     else
-                        lds[lds_slot(q) * G::P + own] = dn_end[k];
+                        lds[lds_slot(q) * G::P + own] = v;
                 }
             });
         }
 #pragma unroll
         for (int i = 0; i < R::JPT; i++)
-            if (jobs[i].lds >= 0) lds[jobs[i].lds] = ring[i];
+            if (jobs[i].lds >= 0)
+                lds[jobs[i].lds + ((jobs[i].lds1 - jobs[i].lds) & -bank_rd)] = job_value(jobs[i], ring[i], z, p.nz);
 
         lds_barrier();
         unroll<1, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            if constexpr (q != 5 && q != 6)
-                g[q] = lds[lds_slot(q) * G::P + own - cyq(q) * G::WP - cxq(q)];
+            constexpr int off = -cyq(q) * G::WP - cxq(q);
+            if constexpr (czq(q) == 1 && q != 5)
+                g[q] = lds[u_addr<Q, TX, TY>(q, bank_rd) + own + off];
+            else if constexpr (q != 5 && q != 6)
+                g[q] = lds[lds_slot(q) * G::P + own + off];
         });
         lds_barrier();
 
         const int bits = node_bits(x, y, z, p.nx, p.ny, p.nz);
         if constexpr (PROBE) {
-            const int64_t d = (int64_t)x + (int64_t)p.nx * ((int64_t)y + (int64_t)p.ny * z);
+            const int d = x + p.nx * (y + p.ny * z);
 #pragma unroll
-            for (int q = 0; q < Q; q++) p.probe_src[d * Q + q] = __float_as_uint(g[q]);
+            for (int q = 0; q < Q; q++) p.probe_src[(int64_t)d * Q + q] = __float_as_uint(g[q]);
             p.probe_type[d] = (uint8_t)bits;
             float mz[10];
 #pragma unroll
@@ -408,9 +434,9 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
             if (bits & 0x40)
                 boundary_moments<Q>(s, bits, p.omega, p.u_lid, m);
             else
-                interior_moments(s, m);
+                interior_moments<Q>(s, m);
             collide(m, p.t_omega, p.omega_h, p.omega);
-            const int64_t i = mom_index<TX, TY>(p, bx, by, z, tid);
+            const int i = mom_index<TX, TY>(p, bx, by, z, tid);
 #pragma unroll
             for (int k = 0; k < 10; k++) Store<ST>::store(p.mom[k], i, m[k], p.enc[k]);
             if constexpr (sizeof(ST) < 4) {
@@ -457,8 +483,7 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
         }
 
         // ---- rotate the plane pipeline
-#pragma unroll
-        for (int k = 0; k < QF; k++) up[k] = up_new[k];
+        p5 = p5_next;
 #pragma unroll
         for (int k = 0; k < 10; k++) {
             mc[k] = mn[k];
