@@ -1,0 +1,147 @@
+"""Benchmark: MLUPS and HBM-roofline fraction of the fused D3Q19 step, 256^3.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: lid-driven cavity, D3Q19, FP32 storage and arithmetic, 256^3 nodes,
+Re 1000, lid speed 0.05, started from rest.  One "step" is one full LBM update
+of every node (one launch of the fused step kernel); the timed region holds K
+steps on the solver's HIP stream, bracketed by hipEvents, with all state
+resident in HBM.  Prints ONE JSON line.
+
+Roofline basis: algorithmic bytes per node update = the 10 stored moments read
+once and written once (80 B in FP32, 40 B in FP16); achieved GB/s = bytes per
+launch / average launch duration.  The CPU baseline is the FP64 oracle
+(oracle/, a scalar C restatement of the reference step) timed on this host on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def measure(n, Q, storage, steps, warmup, fo_mask=0, Re=1000.0, u=0.05, tile=(0, 0, 0)):
+    from cudalbm_amd import Solver
+
+    with Solver(n, n, n, Re, u, velocity_set=Q, storage=storage, tile=tile) as s:
+        if fo_mask:
+            s.fo_enable(fo_mask)
+        if warmup:
+            s.step(warmup)
+        s.synchronize()
+        ms = s.step_timed(steps)
+        tiling = s.tiling
+        mom = s.get_moments() if steps + warmup >= 1 else None
+    nodes = n ** 3
+    mlups = nodes * steps / (ms * 1e-3) / 1e6
+    bpn = 40 if storage == "fp16" else 80
+    gbs = nodes * bpn / (ms * 1e-3 / steps) / 1e9
+    finite = bool(mom is not None and all(abs(float(m.max())) < 1e3 for m in mom))
+    return {"mlups": mlups, "ms_per_step": ms / steps, "achieved_gbs": gbs, "bytes_per_node": bpn,
+            "tiling": list(tiling), "finite": finite}
+
+
+def cpu_baseline(target_s=10.0):
+    """FP64 oracle, D3Q19 96^3 cavity (Re 1000, lid 0.05) from rest, on this
+    host's cores; the step count is sized to about target_s seconds."""
+    from oracle import oracle as O
+
+    n = 96
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    om = O.omega(0.05, 1000.0, n)
+    mom = O.rest_state(n, n, n)
+    t0 = time.perf_counter()
+    O.run(19, n, n, n, om, 0.05, mom, 2, threads)
+    per = (time.perf_counter() - t0) / 2
+    steps = max(2, int(target_s / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    O.run(19, n, n, n, om, 0.05, mom, steps, threads)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, D3Q19 {n}^3 lid cavity Re 1000 u 0.05 from rest, {steps} steps, "
+                      f"{dt:.1f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-variants", action="store_true", help="only the headline D3Q19 FP32 step")
+    ap.add_argument("--only", default="", help="profile one variant: q19fp32|q19fp16|q19fo|q27fp32|q27fp16")
+    ap.add_argument("--tile", default="0,0,0")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 256 MiB device-to-device copy (x4), to calibrate HBM byte counters")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read_per_copy": src.numel() * 4, "copies": 4}), flush=True)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the solver has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    tile = tuple(int(v) for v in args.tile.split(","))
+
+    if args.only:
+        spec = {"q19fp32": (19, "fp32", 0), "q19fp16": (19, "fp16", 0), "q19fo": (19, "fp32", 15),
+                "q27fp32": (27, "fp32", 0), "q27fp16": (27, "fp16", 0)}[args.only]
+        r = measure(args.n, spec[0], spec[1], args.steps, args.warmup, fo_mask=spec[2], tile=tile)
+        print(json.dumps({"variant": args.only, **r}), flush=True)
+        return
+
+    head = measure(args.n, 19, "fp32", args.steps, args.warmup, tile=tile)
+    print(f"# D3Q19 fp32 {args.n}^3: {head['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    variants = {}
+    if not args.no_variants:
+        k = max(50, args.steps // 4)
+        for name, (Q, st, fo) in {"d3q19_fp16": (19, "fp16", 0), "d3q19_fp32_fo_k_kMean_S_SMean": (19, "fp32", 15),
+                                  "d3q27_fp32": (27, "fp32", 0), "d3q27_fp16": (27, "fp16", 0)}.items():
+            r = measure(args.n, Q, st, k, 10, fo_mask=fo)
+            variants[name] = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+                              "roofline_frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4),
+                              "bytes_per_node": r["bytes_per_node"], "tiling": r["tiling"]}
+            print(f"# {name}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": "MLUPS (D3Q19 256^3 lid-driven cavity, FP32)",
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: lid-driven cavity from rest (no dataset)",
+        "config": {"workload": f"lid-driven cavity D3Q19 {args.n}^3 FP32, Re 1000, lid 0.05, rest start",
+                   "velocity_set": "D3Q19", "nodes": args.n ** 3, "tiling": head["tiling"]},
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                     "bytes_per_node": head["bytes_per_node"]},
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -56,20 +56,20 @@ __global__ __launch_bounds__(NT) void residual_partials_kernel(const StepParams 
     __shared__ double sh[NT / 64];
     double ke = 0.0, du = 0.0, uu = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < p.n; i += (int64_t)gridDim.x * NT) {
-        const float rho = 1.0f + Store<ST>::load(p.mom[0], i, p.dec[0]);
-        float u[3];
+        // physical rho and u formed in FP64 from the stored values
+        const double rho = 1.0 + (double)Store<ST>::load(p.mom[0], (int)i, p.dec[0]);
+        double u[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) u[a] = Store<ST>::load(p.mom[1 + a], i, p.dec[1 + a]) * (1.0f / 3.0f);
+        for (int a = 0; a < 3; a++) u[a] = (double)Store<ST>::load(p.mom[1 + a], (int)i, p.dec[1 + a]) / 3.0;
         double uu_i = 0.0, du_i = 0.0;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const double ua = u[a];
-            uu_i += ua * ua;
-            const double d = ua - (double)uprev[a * p.n + i];
+            uu_i += u[a] * u[a];
+            const double d = u[a] - (double)uprev[a * p.n + i];
             du_i += d * d;
-            uprev[a * p.n + i] = u[a];
+            uprev[a * p.n + i] = (float)u[a];
         }
-        ke += 0.5 * (double)rho * uu_i;
+        ke += 0.5 * rho * uu_i;
         uu += uu_i;
         du += first ? uu_i : du_i;
     }

@@ -112,70 +112,114 @@ __device__ __forceinline__ void unroll(F &&f) {
 // Moment vector m: [0] rho-1, [1..3] 3u, [4] mxx, [5] mxy, [6] mxz, [7] myy,
 // [8] myz, [9] mzz (scaled, as stored: velocitySet.cuh:130-142).
 struct Rec {
-    float r3;   // 3 (rho - 1)
-    float rho;  // rho
-    float A;    // -(mxx + myy + mzz)
+    float a;     // 3 rho' - rho (mxx + myy + mzz)
+    float M[10]; // 3 rho m_k, k = 1..9 (M[0] unused)
 };
 
 __device__ __forceinline__ Rec rec_setup(const float (&m)[10]) {
     Rec r;
-    r.r3 = 3.0f * m[0];
-    r.rho = 1.0f + m[0];
-    r.A = -(m[4] + m[7] + m[9]);
+    const float rho = 1.0f + m[0];
+    r.a = fmaf(rho, -(m[4] + m[7] + m[9]), 3.0f * m[0]);
+    const float t = 3.0f * rho;
+#pragma unroll
+    for (int k = 1; k < 10; k++) r.M[k] = t * m[k];
+    r.M[0] = 0.f;
     return r;
 }
 
 // f_q = rho w_q [1 - cs2(mxx+myy+mzz) + c.m + ...] (D3Q19.cuh:346-374), so
-// H_q = Wint_q (3 rho' + rho T_q), T_q = 3 (c_a m_a + c_a^2 m_aa + c_a c_b m_ab) - (mxx+myy+mzz).
+// H_q = Wint_q (a + sum_a c_a M_a + sum_a c_a^2 M_aa + sum_{a<b} c_a c_b M_ab).
+// The even part (diagonal, then off-diagonal terms) and the odd part are formed
+// in one fixed order with a canonical sign, so a pair q / opposite(q) and all
+// populations of one plane share them (common subexpressions).
 template <int Q, int q>
-__device__ __forceinline__ float rec_dev(const Rec &r, const float (&m)[10]) {
+__device__ __forceinline__ float rec_dev(const Rec &r) {
     constexpr int x = cxq(q), y = cyq(q), z = czq(q);
-    float B = 0.f;
-    if constexpr (x != 0) B += float(x) * m[1];
-    if constexpr (y != 0) B += float(y) * m[2];
-    if constexpr (z != 0) B += float(z) * m[3];
-    if constexpr (x != 0) B += m[4];
-    if constexpr (y != 0) B += m[7];
-    if constexpr (z != 0) B += m[9];
-    if constexpr (x * y != 0) B += float(x * y) * m[5];
-    if constexpr (x * z != 0) B += float(x * z) * m[6];
-    if constexpr (y * z != 0) B += float(y * z) * m[8];
-    const float T = fmaf(3.0f, B, r.A);
-    return float(wint<Q>(q)) * fmaf(r.rho, T, r.r3);
+    float e = r.a;
+    if constexpr (x != 0) e += r.M[4];
+    if constexpr (y != 0) e += r.M[7];
+    if constexpr (z != 0) e += r.M[9];
+    // off-diagonal part, canonical sign: first nonzero product positive
+    constexpr int pxy = x * y, pxz = x * z, pyz = y * z;
+    constexpr int s_off = pxy != 0 ? pxy : (pxz != 0 ? pxz : (pyz != 0 ? pyz : 1));
+    if constexpr (pxy != 0 || pxz != 0 || pyz != 0) {
+        float od = 0.f;
+        bool first = true;
+        if constexpr (pxy != 0) { od = r.M[5]; first = false; }
+        if constexpr (pxz != 0) { od = first ? r.M[6] : (pxz * s_off > 0 ? od + r.M[6] : od - r.M[6]); first = false; }
+        if constexpr (pyz != 0) { od = first ? r.M[8] : (pyz * s_off > 0 ? od + r.M[8] : od - r.M[8]); }
+        e = s_off > 0 ? e + od : e - od;
+    }
+    // odd part, canonical sign
+    constexpr int s_odd = x != 0 ? x : (y != 0 ? y : (z != 0 ? z : 1));
+    float h = e;
+    if constexpr (x != 0 || y != 0 || z != 0) {
+        float o = 0.f;
+        bool first = true;
+        if constexpr (x != 0) { o = r.M[1]; first = false; }
+        if constexpr (y != 0) { o = first ? r.M[2] : (y * s_odd > 0 ? o + r.M[2] : o - r.M[2]); first = false; }
+        if constexpr (z != 0) { o = first ? r.M[3] : (z * s_odd > 0 ? o + r.M[3] : o - r.M[3]); }
+        h = s_odd > 0 ? e + o : e - o;
+    }
+    constexpr int W = wint<Q>(q);
+    if constexpr (W == 1)
+        return h;
+    else
+        return float(W) * h;
 }
 
-// Raw second-order sums of H.
+// Raw second-order sums of H, accumulated over opposite pairs (q, q+1):
+// P = H_q + H_q+1 feeds the even sums, Mi = H_q - H_q+1 the odd ones.
 struct Sums {
     float s0, sx, sy, sz, sxx, sxy, sxz, syy, syz, szz;
 };
 
 template <int Q>
 __device__ __forceinline__ Sums pop_sums(const float (&g)[27]) {
-    Sums s{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    unroll<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
+    Sums s{g[0], 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unroll<0, (Q - 1) / 2>([&](auto pc) {
+        constexpr int q = 1 + 2 * decltype(pc)::value;  // q and q+1 are opposite
         constexpr int x = cxq(q), y = cyq(q), z = czq(q);
-        const float v = g[q];
-        s.s0 += v;
-        if constexpr (x != 0) s.sx += float(x) * v;
-        if constexpr (y != 0) s.sy += float(y) * v;
-        if constexpr (z != 0) s.sz += float(z) * v;
-        if constexpr (x != 0) s.sxx += v;
-        if constexpr (y != 0) s.syy += v;
-        if constexpr (z != 0) s.szz += v;
-        if constexpr (x * y != 0) s.sxy += float(x * y) * v;
-        if constexpr (x * z != 0) s.sxz += float(x * z) * v;
-        if constexpr (y * z != 0) s.syz += float(y * z) * v;
+        static_assert(cxq(q + 1) == -x && cyq(q + 1) == -y && czq(q + 1) == -z, "pair order");
+        const float P = g[q] + g[q + 1];
+        const float Mi = g[q] - g[q + 1];
+        s.s0 += P;
+        if constexpr (x != 0) s.sx = x > 0 ? s.sx + Mi : s.sx - Mi;
+        if constexpr (y != 0) s.sy = y > 0 ? s.sy + Mi : s.sy - Mi;
+        if constexpr (z != 0) s.sz = z > 0 ? s.sz + Mi : s.sz - Mi;
+        if constexpr (x != 0) s.sxx += P;
+        if constexpr (y != 0) s.syy += P;
+        if constexpr (z != 0) s.szz += P;
+        if constexpr (x * y != 0) s.sxy = x * y > 0 ? s.sxy + P : s.sxy - P;
+        if constexpr (x * z != 0) s.sxz = x * z > 0 ? s.sxz + P : s.sxz - P;
+        if constexpr (y * z != 0) s.syz = y * z > 0 ? s.syz + P : s.syz - P;
     });
     return s;
 }
 
 // Interior moments (velocitySet.cuh:398-417) followed by scale (velocitySet.cuh:130-142):
 // rho' = S0/3D, R = 3D rho = 3D + S0, 3u = 3 Sx/R, mxx = (4.5 Sxx - 1.5 S0)/R, mxy = 9 Sxy/R.
+// x / 3D without a systematic bias: 1/3D split into a float head and tail
+// (together exact to ~2^-48), so the product carries at most ~1 ulp of
+// round-off that does not repeat identically step after step.
+template <int Q>
+__device__ __forceinline__ float div_d3(float x) {
+    constexpr double inv = Q == 19 ? 1.0 / 108.0 : 1.0 / 648.0;
+    constexpr float hi = float(inv);
+    constexpr float lo = float(inv - double(hi));
+    return fmaf(x, hi, x * lo);
+}
+// 1/r from the hardware reciprocal plus one Newton step (~0.5 ulp).
+__device__ __forceinline__ float recip(float r) {
+    float y = __builtin_amdgcn_rcpf(r);
+    const float e = fmaf(-r, y, 1.0f);
+    return fmaf(y, e, y);
+}
+
 template <int Q>
 __device__ __forceinline__ void interior_moments(const Sums &s, float (&m)[10]) {
-    const float inv = 1.0f / (d3<Q>() + s.s0);
-    m[0] = s.s0 / d3<Q>();
+    const float inv = recip(d3<Q>() + s.s0);
+    m[0] = div_d3<Q>(s.s0);
     m[1] = 3.0f * s.sx * inv;
     m[2] = 3.0f * s.sy * inv;
     m[3] = 3.0f * s.sz * inv;

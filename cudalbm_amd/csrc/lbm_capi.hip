@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/lbm_amd.h"
+#include "lbm_amd.h"
 #include "aux_kernels.hpp"
 #include "lbm_internal.hpp"
 
@@ -136,8 +136,11 @@ int choose_lz(int nx, int ny, int nz, int tx, int ty) {
     return best > 0 ? best : nz;
 }
 
-void choose_tile(int nx, int ny, int *tx, int *ty) {
-    if (nx % 16 == 0 && ny % 16 == 0) { *tx = 16; *ty = 16; }
+void choose_tile(int Q, int nx, int ny, int *tx, int *ty) {
+    // measured on MI355X at 256^3: D3Q19 is fastest with 32x16 footprints (8 waves,
+    // fewest face words per node); D3Q27's LDS allows 16x16 at 3 workgroups/CU
+    if (Q == 19 && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
+    else if (nx % 16 == 0 && ny % 16 == 0) { *tx = 16; *ty = 16; }
     else if (nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
     else { *tx = 8; *ty = 8; }
 }
@@ -442,7 +445,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     if (!(c.Re > 0) || !std::isfinite(c.u_inf)) return bad(LBM_ERR_ARG, "Re must be > 0 and u_inf finite");
     h->Q = c.velocity_set;
     if (c.tile_x == 0 || c.tile_y == 0)
-        choose_tile(c.nx, c.ny, &h->TX, &h->TY);
+        choose_tile(h->Q, c.nx, c.ny, &h->TX, &h->TY);
     else {
         h->TX = c.tile_x;
         h->TY = c.tile_y;
@@ -457,7 +460,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     h->nby = c.ny / h->TY;
     h->nc = c.nz / h->LZ;
     h->n = (int64_t)c.nx * c.ny * c.nz;
-    if (h->n * 32 + 31 >= (int64_t)UINT32_MAX) return bad(LBM_ERR_SIZE, "mesh too large for 32-bit probe ids");
+    if (h->n >= (int64_t)INT32_MAX) return bad(LBM_ERR_SIZE, "mesh too large: node indices are 32-bit");
     // latticeMesh.cuh:178-182 (computed in FP64 here)
     const double nu = c.u_inf * (double)(c.nx - 1) / c.Re;
     h->tau = 0.5 + 3.0 * nu;
@@ -475,6 +478,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
             HIPX(hipMemsetAsync(h->mom[i], 0, h->elem() * h->n, h->stream));
         }
         h->fwords = face_words(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY, h->LZ, &h->yoff, &h->zoff);
+        if (h->fwords >= (int64_t)INT32_MAX) throw ArgError{LBM_ERR_SIZE, "face buffers exceed 2^31 words"};
         for (int s = 0; s < 2; s++) {
             HIPX(hipMalloc(&h->faces[s], sizeof(float) * h->fwords));
             HIPX(hipMemsetAsync(h->faces[s], 0, sizeof(float) * h->fwords, h->stream));
@@ -716,6 +720,7 @@ int lbm_reduce(lbm_handle *h, double *ke, double *res) {
 int lbm_probe_streaming(lbm_handle *h, uint32_t *src_out, uint8_t *type_out) {
     return guarded(h, [&] {
         if (!src_out || !type_out) throw ArgError{LBM_ERR_ARG, "null output"};
+        if (h->n * 32 + 31 >= (int64_t)UINT32_MAX) throw ArgError{LBM_ERR_SIZE, "mesh too large for 32-bit probe ids"};
         HIPX(hipStreamSynchronize(h->stream));
         const int64_t n = h->n;
         uint32_t *dsrc = nullptr;

@@ -127,12 +127,17 @@ template <int Q, int TX, int TY>
 __host__ __device__ constexpr int u_addr(int q, int bank) {
     return bank ? (VS<Q>::NSLOT + u_index<Q>(q)) * Geo<TX, TY>::P : lds_slot(q) * Geo<TX, TY>::P;
 }
+// Ring job in three registers: LDS word offsets of bank 0 / bank 1 (16 bits
+// each; `lds` = -1 when the thread has no job), the face-set offset without the
+// plane term (-1 when outside the domain), and plane stride * 4 + (dz + 1).
 struct Job {
-    int lds;  // LDS word offset (bank 0), or -1 when the thread has no job
-    int lds1; // LDS word offset when up-going pops use bank 1
-    int gb;   // face-set offset without the plane term, or -1 when outside the domain
-    int zs;   // plane stride inside the face block
-    int dz;   // source plane offset: -cz
+    int lds;  // bank0 | bank1 << 16, or -1
+    int gb;
+    int zsd;
+    __device__ __forceinline__ int lds0() const { return lds & 0xFFFF; }
+    __device__ __forceinline__ int lds1() const { return (lds >> 16) & 0xFFFF; }
+    __device__ __forceinline__ int zs() const { return zsd >> 2; }
+    __device__ __forceinline__ int dz() const { return (zsd & 3) - 1; }
 };
 
 // Ring job j of this workgroup: (side, face slot k, position) decoded against the
@@ -141,7 +146,7 @@ template <int Q, int TX, int TY>
 __device__ __forceinline__ Job make_job(int j, const StepParams &p, int bx, int by) {
     using G = Geo<TX, TY>;
     constexpr int QF = VS<Q>::QF;
-    Job job{-1, -1, -1, 0, 0};
+    Job job{-1, -1, 0};
     int start = 0;
     unroll<0, 4>([&](auto sc) {
         constexpr int side = decltype(sc)::value;
@@ -162,20 +167,21 @@ __device__ __forceinline__ Job make_job(int j, const StepParams &p, int bx, int 
                     rx = pos + (cxq(q) < 0 ? 1 : 0);  // rx in [max(0,-cx), min(TX-1, TX-1-cx)]
                 }
                 const int cell = (ry + 1) * G::WP + (rx + 1);
-                job.lds = lds_slot(q) * G::P + cell;
-                job.lds1 = czq(q) == 1 ? u_addr<Q, TX, TY>(q, 1) + cell : job.lds;
-                job.dz = -czq(q);
+                const int l0 = lds_slot(q) * G::P + cell;
+                const int l1 = czq(q) == 1 ? u_addr<Q, TX, TY>(q, 1) + cell : l0;
+                job.lds = l0 | (l1 << 16);
+                constexpr int dzq = -czq(q);
                 const int gx = bx * TX + rx, gy = by * TY + ry;
                 const bool inside = gx >= 0 && gx < p.nx && gy >= 0 && gy < p.ny;
                 if constexpr (side < 2) {
                     const int col = side == 0 ? bx - 1 : bx + 1;
                     constexpr int s_face = side == 0 ? 1 : 0;  // east (1) / west (0) faces
-                    job.zs = p.ny;
+                    job.zsd = p.ny * 4 + (dzq + 1);
                     job.gb = inside ? (((s_face * p.nbx + col) * QF + k) * p.nz) * p.ny + gy : -1;
                 } else {
                     const int row = side == 2 ? by - 1 : by + 1;
                     constexpr int s_face = side == 2 ? 1 : 0;
-                    job.zs = p.nx;
+                    job.zsd = p.nx * 4 + (dzq + 1);
                     job.gb = inside ? p.yoff + (((s_face * p.nby + row) * QF + k) * p.nz) * p.nx + gx : -1;
                 }
             }
@@ -188,13 +194,13 @@ __device__ __forceinline__ Job make_job(int j, const StepParams &p, int bx, int 
 // Raw face word for destination plane zdest (address clamped into the buffer so
 // the load is unconditional); job_value() applies the validity afterwards.
 __device__ __forceinline__ float job_raw(const Job &j, const float *__restrict__ fin, int zdest, int nz) {
-    int zs = zdest + j.dz;
+    int zs = zdest + j.dz();
     zs = zs < 0 ? 0 : (zs >= nz ? nz - 1 : zs);
-    const int off = j.gb < 0 ? 0 : j.gb + zs * j.zs;
+    const int off = j.gb < 0 ? 0 : j.gb + zs * j.zs();
     return fin[off];
 }
 __device__ __forceinline__ float job_value(const Job &j, float raw, int zdest, int nz) {
-    const int zs = zdest + j.dz;
+    const int zs = zdest + j.dz();
     return (j.gb >= 0 && zs >= 0 && zs < nz) ? raw : 0.f;
 }
 
@@ -212,7 +218,6 @@ template <int Q>
 __device__ __forceinline__ int zf_off(const StepParams &p, int s, int c, int k, int y, int x) {
     return p.zoff + (((s * p.nc + c) * VS<Q>::QF + k) * p.ny + y) * p.nx + x;
 }
-
 // Population q of node (x,y,z) rebuilt from its moments (or its probe id).
 template <int Q, int q, bool PROBE>
 __device__ __forceinline__ float pop_of(const Rec &r, const float (&m)[10], int x, int y, int z,
@@ -220,14 +225,14 @@ __device__ __forceinline__ float pop_of(const Rec &r, const float (&m)[10], int 
     if constexpr (PROBE)
         return probe_id(x, y, z, p.nx, p.ny, q);
     else
-        return rec_dev<Q, q>(r, m);
+        return rec_dev<Q, q>(r);
 }
 
-// Outgoing face populations of one node after collision (or its probe ids).
+// Outgoing x/y face populations of one node after collision (or its probe ids).
 template <int Q, int TX, int TY, bool PROBE>
-__device__ __forceinline__ void write_faces(const StepParams &p, float *__restrict__ fout,
-                                            const float (&m)[10], int tx, int ty, int x, int y,
-                                            int z, int bx, int by, int c, int z0, int z1) {
+__device__ __forceinline__ void write_xy_faces(const StepParams &p, float *__restrict__ fout,
+                                               const float (&m)[10], int tx, int ty, int x, int y,
+                                               int z, int bx, int by) {
     constexpr int QF = VS<Q>::QF;
     const Rec r = rec_setup(m);
     if (tx == 0 && bx > 0)
@@ -254,6 +259,16 @@ __device__ __forceinline__ void write_faces(const StepParams &p, float *__restri
             constexpr int q = face_q<Q>(1, 1, k);
             fout[yf_off<Q>(p, 1, by, k, z, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
         });
+}
+
+// Outgoing z face populations at the chunk's first and last planes.
+template <int Q, bool PROBE>
+__device__ __forceinline__ void write_z_faces(const StepParams &p, float *__restrict__ fout,
+                                              const float (&m)[10], int x, int y, int z, int c, int z0,
+                                              int z1) {
+    constexpr int QF = VS<Q>::QF;
+    if (!((z == z0 && c > 0) || (z == z1 && c < p.nc - 1))) return;
+    const Rec r = rec_setup(m);
     if (z == z0 && c > 0)
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
@@ -268,17 +283,85 @@ __device__ __forceinline__ void write_faces(const StepParams &p, float *__restri
         });
 }
 
+template <int Q, int TX, int TY, bool PROBE>
+__device__ __forceinline__ void write_faces(const StepParams &p, float *__restrict__ fout,
+                                            const float (&m)[10], int tx, int ty, int x, int y,
+                                            int z, int bx, int by, int c, int z0, int z1) {
+    write_xy_faces<Q, TX, TY, PROBE>(p, fout, m, tx, ty, x, y, z, bx, by);
+    write_z_faces<Q, PROBE>(p, fout, m, x, y, z, c, z0, z1);
+}
+
+// Edge nodes of the footprint (the ones that own x/y face populations),
+// numbered bottom row, top row, left column, right column (corners once).
 template <int TX, int TY>
-__device__ __forceinline__ int mom_index(const StepParams &p, int bx, int by, int z, int tid) {
-    return (((z * p.nby + by) * p.nbx + bx) * (TX * TY)) + tid;
+struct Edge {
+    static constexpr int NE = 2 * TX + 2 * (TY - 2);
+    __device__ static __forceinline__ int index(int tx, int ty) {
+        if (ty == 0) return tx;
+        if (ty == TY - 1) return TX + tx;
+        if (tx == 0) return 2 * TX + ty - 1;
+        if (tx == TX - 1) return 2 * TX + (TY - 2) + ty - 1;
+        return -1;
+    }
+    __device__ static __forceinline__ void coords(int e, int &tx, int &ty) {
+        if (e < TX) { tx = e; ty = 0; }
+        else if (e < 2 * TX) { tx = e - TX; ty = TY - 1; }
+        else if (e < 2 * TX + TY - 2) { tx = 0; ty = e - 2 * TX + 1; }
+        else { tx = TX - 1; ty = e - 2 * TX - (TY - 2) + 1; }
+    }
+};
+
+// x/y face populations of one finished plane, written cooperatively: edge
+// nodes park their post-collision moments in LDS (edge_m, 10 x NE words) and
+// the first NE threads -- one or two whole waves -- rebuild and store the face
+// populations, instead of every wave rebuilding them for its few edge lanes.
+template <int Q, int TX, int TY, bool PROBE>
+__device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__restrict__ fout,
+                                               const float *edge_m, int tid, int bx, int by, int z) {
+    using E = Edge<TX, TY>;
+    if (tid >= E::NE) return;
+    int tx, ty;
+    E::coords(tid, tx, ty);
+    float m[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) m[k] = edge_m[k * E::NE + tid];
+    write_xy_faces<Q, TX, TY, PROBE>(p, fout, m, tx, ty, bx * TX + tx, by * TY + ty, z, bx, by);
+}
+
+// First node of this workgroup's tile in plane z (uniform); add the thread index.
+template <int TX, int TY>
+__device__ __forceinline__ unsigned plane_base(const StepParams &p, int bx, int by, int z) {
+    return (unsigned)(((z * p.nby + by) * p.nbx + bx) * (TX * TY));
 }
 
 template <typename ST, int TX, int TY>
 __device__ __forceinline__ void load_plane(const StepParams &p, int bx, int by, int z, int tid,
                                            float (&m)[10]) {
-    const int i = mom_index<TX, TY>(p, bx, by, z, tid);
+    const int i = (int)plane_base<TX, TY>(p, bx, by, z) + tid;
 #pragma unroll
     for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], i, p.dec[k]);
+}
+
+// Next plane's old moments, or past the chunk's last plane the down-going
+// populations of the plane above it (bottom face of chunk c+1; zero above the
+// domain) in m[0..QF-1].
+template <int Q, typename ST, int TX, int TY, bool PROBE>
+__device__ __forceinline__ void load_next(const StepParams &p, const float *__restrict__ fin, int bx, int by,
+                                          int c, int zn, int z1, int tid, int x, int y, float (&m)[10]) {
+    if (zn <= z1) {
+        if constexpr (!PROBE) {
+            load_plane<ST, TX, TY>(p, bx, by, zn, tid, m);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 10; k++) m[k] = 0.f;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < VS<Q>::QF; k++)
+            m[k] = c < p.nc - 1 ? fin[zf_off<Q>(p, 0, c + 1, k, y, x)] : 0.f;
+#pragma unroll
+        for (int k = VS<Q>::QF; k < 10; k++) m[k] = 0.f;
+    }
 }
 
 // The fused step.  PROBE replaces the moment algebra by integer ids so that
@@ -288,16 +371,18 @@ __device__ __forceinline__ void load_plane(const StepParams &p, int bx, int by, 
 // this plane, up-going pops of the previous plane in bank 0), NU more slots for
 // the second bank of up-going pops (the bank alternates with the plane parity so
 // plane z can stage its own up-going pops while plane z-1's are being read), and
-// QF*NT words holding the down-going pops of the plane above the chunk.
+// two parity buffers of the edge nodes' moments for the deferred face writes.
 template <int Q, int TX, int TY, typename ST, bool FO, bool PROBE>
-__global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
+__global__ __launch_bounds__(TX *TY, Q == 19 ? 4 : 3) void step_kernel(const StepParams p) {
     using G = Geo<TX, TY>;
     using R = Ring<Q, TX, TY>;
     constexpr int QF = VS<Q>::QF;
     constexpr int NS = VS<Q>::NSLOT;
     constexpr int NU = VS<Q>::NU;
-    __shared__ float lds[(NS + NU) * G::P + QF * G::NT];
-    float *const dnst = lds + (NS + NU) * G::P;
+    using E = Edge<TX, TY>;
+    __shared__ float lds[(NS + NU) * G::P + 2 * 10 * E::NE];
+    float *const edge_m = lds + (NS + NU) * G::P;  // [plane parity][10][NE]
+    const int my_edge = E::index(threadIdx.x % TX, threadIdx.x / TX);
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -332,27 +417,29 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
             }
         });
     }
-    // Down-going populations of plane z1+1 (bottom face of the chunk above).
-#pragma unroll
-    for (int k = 0; k < QF; k++) dnst[k * G::NT + tid] = c < p.nc - 1 ? fin[zf_off<Q>(p, 0, c + 1, k, y, x)] : 0.f;
-
+    // mc: old moments of plane z; mn: of plane z+1, or -- on the chunk's last
+    // plane -- the down-going populations of plane z1+1 (bottom face of the chunk
+    // above, zero above the domain), fetched into the idle prefetch slot.
     float mc[10], mn[10];
     if constexpr (!PROBE) {
         load_plane<ST, TX, TY>(p, bx, by, z0, tid, mc);
-        load_plane<ST, TX, TY>(p, bx, by, z0 + 1 <= z1 ? z0 + 1 : z1, tid, mn);
     } else {
 #pragma unroll
-        for (int k = 0; k < 10; k++) mc[k] = mn[k] = 0.f;
+        for (int k = 0; k < 10; k++) mc[k] = 0.f;
     }
+    load_next<Q, ST, TX, TY, PROBE>(p, fin, bx, by, c, z0 + 1, z1, tid, x, y, mn);
     float ring[R::JPT];
 #pragma unroll
     for (int i = 0; i < R::JPT; i++) ring[i] = job_raw(jobs[i], fin, z0, p.nz);
 
     for (int z = z0; z <= z1; z++) {
+        // ---- x/y faces of plane z-2: its edge moments were parked two iterations
+        //      ago; that parity buffer is rewritten only after this plane's barriers
+        if (z - 2 >= z0) flush_xy_faces<Q, TX, TY, PROBE>(p, fout, edge_m + (z & 1) * 10 * E::NE, tid, bx, by, z - 2);
         // ---- prefetch plane z+2 and the ring words of plane z+1 (unconditional
         //      loads of clamped addresses; unused values are discarded)
         float mq[10];
-        if constexpr (!PROBE) load_plane<ST, TX, TY>(p, bx, by, z + 2 <= z1 ? z + 2 : z1, tid, mq);
+        if (z + 2 <= z1 + 1) load_next<Q, ST, TX, TY, PROBE>(p, fin, bx, by, c, z + 2, z1, tid, x, y, mq);
         float ring_n[R::JPT];
 #pragma unroll
         for (int i = 0; i < R::JPT; i++) ring_n[i] = job_raw(jobs[i], fin, z + 1, p.nz);
@@ -394,7 +481,7 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
             unroll<1, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 if constexpr (czq(q) == -1) {
-                    const float v = dnst[face_k<Q>(2, q) * G::NT + tid];
+                    const float v = mn[face_k<Q>(2, q)];
                     if constexpr (q == 6)
                         g[6] = v;
                     else
@@ -405,7 +492,7 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
 #pragma unroll
         for (int i = 0; i < R::JPT; i++)
             if (jobs[i].lds >= 0)
-                lds[jobs[i].lds + ((jobs[i].lds1 - jobs[i].lds) & -bank_rd)] = job_value(jobs[i], ring[i], z, p.nz);
+                lds[bank_rd ? jobs[i].lds1() : jobs[i].lds0()] = job_value(jobs[i], ring[i], z, p.nz);
 
         lds_barrier();
         unroll<1, Q>([&](auto qc) {
@@ -427,7 +514,7 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
             float mz[10];
 #pragma unroll
             for (int k = 0; k < 10; k++) mz[k] = 0.f;
-            write_faces<Q, TX, TY, true>(p, fout, mz, tx, ty, x, y, z, bx, by, c, z0, z1);
+
         } else {
             const Sums s = pop_sums<Q>(g);
             float m[10];
@@ -436,17 +523,16 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
             else
                 interior_moments<Q>(s, m);
             collide(m, p.t_omega, p.omega_h, p.omega);
-            const int i = mom_index<TX, TY>(p, bx, by, z, tid);
+            const unsigned pb = plane_base<TX, TY>(p, bx, by, z);
+            const unsigned i = pb + (unsigned)tid;
 #pragma unroll
-            for (int k = 0; k < 10; k++) Store<ST>::store(p.mom[k], i, m[k], p.enc[k]);
-            if constexpr (sizeof(ST) < 4) {
+            for (int k = 0; k < 10; k++) Store<ST>::store(p.mom[k], (int)i, m[k], p.enc[k]);
+            if (my_edge >= 0) {
                 // neighbours must see the same (rounded) moments the next step reloads
-                float mr[10];
+                float *em = edge_m + (z & 1) * 10 * E::NE;
 #pragma unroll
-                for (int k = 0; k < 10; k++) mr[k] = float(ST(m[k] * p.enc[k])) * p.dec[k];
-                write_faces<Q, TX, TY, false>(p, fout, mr, tx, ty, x, y, z, bx, by, c, z0, z1);
-            } else {
-                write_faces<Q, TX, TY, false>(p, fout, m, tx, ty, x, y, z, bx, by, c, z0, z1);
+                for (int k = 0; k < 10; k++)
+                    em[k * E::NE + my_edge] = sizeof(ST) < 4 ? float(ST(m[k] * p.enc[k])) * p.dec[k] : m[k];
             }
             if constexpr (FO) {
                 // kineticEnergy.cuh:71-85 and strainRateTensor.cuh:71-84 on the
@@ -487,10 +573,29 @@ __global__ __launch_bounds__(TX *TY) void step_kernel(const StepParams p) {
 #pragma unroll
         for (int k = 0; k < 10; k++) {
             mc[k] = mn[k];
-            if constexpr (!PROBE) mn[k] = mq[k];
+            mn[k] = mq[k];
         }
 #pragma unroll
         for (int i = 0; i < R::JPT; i++) ring[i] = ring_n[i];
+    }
+    // faces of the chunk's last two planes
+    lds_barrier();
+    if (z1 - 1 >= z0) flush_xy_faces<Q, TX, TY, PROBE>(p, fout, edge_m + ((z1 - 1) & 1) * 10 * E::NE, tid, bx, by, z1 - 1);
+    flush_xy_faces<Q, TX, TY, PROBE>(p, fout, edge_m + (z1 & 1) * 10 * E::NE, tid, bx, by, z1);
+    // z faces of the chunk's end planes, from the moments this thread stored
+    // (re-read: keeping them live through the plane loop costs more registers
+    // than 2/lz of a plane's reads)
+    if (c > 0) {
+        float m[10];
+        if constexpr (!PROBE) load_plane<ST, TX, TY>(p, bx, by, z0, tid, m);
+        else for (int k = 0; k < 10; k++) m[k] = 0.f;
+        write_z_faces<Q, PROBE>(p, fout, m, x, y, z0, c, z0, z1 + 1);
+    }
+    if (c < p.nc - 1) {
+        float m[10];
+        if constexpr (!PROBE) load_plane<ST, TX, TY>(p, bx, by, z1, tid, m);
+        else for (int k = 0; k < 10; k++) m[k] = 0.f;
+        write_z_faces<Q, PROBE>(p, fout, m, x, y, z1, c, z0 - 1, z1);
     }
 }
 

@@ -1,0 +1,124 @@
+"""GPU tests of the fused function objects, FP64 reductions and FP16 storage.
+
+Tolerances:
+  * k, S (FP32, from FP32 moments): relative 1e-6 against the same formulas
+    evaluated in FP64 on the moments the GPU returned;
+  * kMean, SMean (FP64 accumulators of FP32 samples): 1e-9 relative against an
+    FP64 running mean of the per-step FP32 fields read back after every step;
+  * kinetic energy / residual: 1e-9 relative against FP64 host sums of the same
+    fields;
+  * FP16 scaled-moment storage: velocity rel-L2 <= 1e-3 against the FP64 oracle
+    after 500 steps (D3Q19 48^3, Re 400).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_function_objects_instantaneous(native_lib):
+    from cudalbm_amd import Solver
+    from cudalbm_amd import native as N
+
+    n = 32
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=19) as s:
+        s.fo_enable(N.LBM_FO_K | N.LBM_FO_S)
+        s.step(40)
+        mom = [m.astype(np.float64) for m in s.get_moments()]
+        k = s.fo_get(N.LBM_FIELD_K)
+        S = s.fo_get(N.LBM_FIELD_S)
+        tau = s.tau
+    kr = O.k_field(mom)
+    Sr = O.S_field(mom, tau)
+    assert np.allclose(k, kr, rtol=1e-6, atol=1e-9)
+    assert np.allclose(S, Sr, rtol=1e-5, atol=1e-8)
+    assert np.abs(k).max() > 1e-3
+
+
+def test_function_objects_means(native_lib):
+    from cudalbm_amd import Solver
+    from cudalbm_amd import native as N
+
+    n = 32
+    steps = 12
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=19) as s:
+        s.fo_enable(N.LBM_FO_K | N.LBM_FO_KMEAN | N.LBM_FO_S | N.LBM_FO_SMEAN)
+        kmean = np.zeros(n ** 3)
+        smean = np.zeros((6, n ** 3))
+        for t in range(steps):
+            s.step(1)
+            k = s.fo_get(N.LBM_FIELD_K)
+            S = s.fo_get(N.LBM_FIELD_S)
+            kmean = kmean + (k - kmean) / (t + 1)
+            smean = smean + (S - smean) / (t + 1)
+        km = s.fo_get(N.LBM_FIELD_KMEAN)
+        sm = s.fo_get(N.LBM_FIELD_SMEAN)
+    assert np.allclose(km, kmean, rtol=1e-9, atol=1e-15)
+    assert np.allclose(sm, smean, rtol=1e-9, atol=1e-15)
+
+
+def test_reductions_fp64(native_lib):
+    from cudalbm_amd import Solver
+
+    n = 32
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=19) as s:
+        s.step(30)
+        ke1, r1 = s.reduce()
+        m1 = [m.astype(np.float64) for m in s.get_moments()]
+        s.step(10)
+        ke2, r2 = s.reduce()
+        m2 = [m.astype(np.float64) for m in s.get_moments()]
+        ke3, r3 = s.reduce()
+    u1 = np.stack(m1[1:4]) / 3.0
+    u2 = np.stack(m2[1:4]) / 3.0
+    ke_ref = 0.5 * np.sum((1.0 + m2[0]) * np.sum(u2 ** 2, axis=0))
+    # the previous field is kept in FP32
+    r_ref = np.linalg.norm(u2 - u1.astype(np.float32)) / np.linalg.norm(u2)
+    assert r1 == 1.0
+    assert abs(ke2 - ke_ref) <= 1e-9 * ke_ref
+    assert abs(r2 - r_ref) <= 1e-6 * r_ref
+    assert r3 <= 1e-7  # nothing moved since the last call (previous field kept in FP32)
+    assert ke3 == ke2
+
+
+def test_fp16_storage_accuracy(native_lib):
+    from cudalbm_amd import Solver
+
+    n, Re, u, steps = 48, 400.0, 0.05, 500
+    ref = O.run(19, n, n, n, O.omega(u, Re, n), u, O.rest_state(n, n, n), steps)
+    with Solver(n, n, n, Re, u, velocity_set=19, storage="fp16") as s:
+        s.step(steps)
+        got = s.get_moments()
+    vr = np.stack(ref[1:4])
+    vg = np.stack(got[1:4]).astype(np.float64)
+    err = np.linalg.norm(vg - vr) / np.linalg.norm(vr)
+    print(f"fp16 velocity rel-L2 {err:.3e}", flush=True)
+    assert err <= 1e-3
+
+
+def test_layout_tiled8_round_trip(native_lib):
+    """Upload in the reference 8^3-tiled order, read back in both orders."""
+    from cudalbm_amd import Solver
+    from cudalbm_amd import native as N
+
+    nx, ny, nz = 32, 16, 24
+    rng = np.random.default_rng(5)
+    g = [rng.normal(scale=1e-3, size=nx * ny * nz).astype(np.float32) for _ in range(10)]
+    # global -> tiled8 on the host
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    t8 = (x % 8) + 8 * ((y % 8) + 8 * ((z % 8) + 8 * ((x // 8) + (nx // 8) * ((y // 8) + (ny // 8) * (z // 8)))))
+    tiled = []
+    for a in g:
+        b = np.empty_like(a)
+        b[t8] = a
+        tiled.append(b)
+    with Solver(nx, ny, nz, 100.0, 0.05, tile=(16, 16, 4)) as s:
+        s.set_moments(tiled, layout=N.LBM_LAYOUT_TILED8)
+        back_g = s.get_moments(layout=N.LBM_LAYOUT_GLOBAL)
+        back_t = s.get_moments(layout=N.LBM_LAYOUT_TILED8)
+    for k in range(10):
+        assert np.array_equal(back_g[k], g[k])
+        assert np.array_equal(back_t[k], tiled[k])
