@@ -60,7 +60,25 @@ def build(verbose: bool = False) -> str:
     cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_apps(verbose)
     return LIB
+
+
+APPS = {"momentBasedD3Q19": "momentBasedD3Q19.cpp", "momentBasedD3Q27": "momentBasedD3Q27.cpp"}
+BIN = os.path.join(HERE, "bin")
+
+
+def build_apps(verbose: bool = False) -> None:
+    """Solver drivers, linked against liblbm_amd.so (found via $ORIGIN/..)."""
+    os.makedirs(BIN, exist_ok=True)
+    apps = os.path.join(CSRC, "apps")
+    for exe, main in APPS.items():
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+               os.path.join(apps, "momentBased.cpp"), os.path.join(apps, main),
+               "-L", HERE, "-llbm_amd", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(BIN, exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

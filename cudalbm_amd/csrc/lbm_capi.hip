@@ -645,6 +645,15 @@ int lbm_fo_enable(lbm_handle *h, unsigned mask) {
         if ((mask & LBM_FO_S) && !h->S) HIPX(hipMalloc(&h->S, sizeof(float) * 6 * n));
         if ((mask & LBM_FO_KMEAN) && !h->kmean) HIPX(hipMalloc(&h->kmean, sizeof(double) * n));
         if ((mask & LBM_FO_SMEAN) && !h->smean) HIPX(hipMalloc(&h->smean, sizeof(double) * 6 * n));
+        // instantaneous fields start at zero too (objectAllocator allocates them zeroed)
+        if (mask & LBM_FO_K) {
+            hipLaunchKernelGGL(fill_kernel_f32, dim3(2048), dim3(256), 0, h->stream, h->k, n, 0.0f);
+            HIPX(hipGetLastError());
+        }
+        if (mask & LBM_FO_S) {
+            hipLaunchKernelGGL(fill_kernel_f32, dim3(2048), dim3(256), 0, h->stream, h->S, 6 * n, 0.0f);
+            HIPX(hipGetLastError());
+        }
         if (mask & LBM_FO_KMEAN) {
             hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, h->kmean, n, 0.0);
             HIPX(hipGetLastError());

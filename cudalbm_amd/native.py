@@ -214,3 +214,104 @@ class Solver:
         self._check(lib().lbm_probe_streaming(self._h, src.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                               typ.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
         return src.reshape(self.n, self.Q), typ
+
+
+# ------------------------------------------------------------------ case I/O
+class lbm_case_info(ctypes.Structure):
+    _fields_ = [
+        ("case_name", ctypes.c_char * 256),
+        ("Re", ctypes.c_double),
+        ("u_inf", ctypes.c_double),
+        ("n_time_steps", ctypes.c_int64),
+        ("save_interval", ctypes.c_int64),
+        ("info_interval", ctypes.c_int64),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("Lx", ctypes.c_double),
+        ("Ly", ctypes.c_double),
+        ("Lz", ctypes.c_double),
+        ("latest_time", ctypes.c_int64),
+    ]
+
+
+EXPORTED += ["lbm_case_last_error", "lbm_case_read", "lbm_case_parameter", "lbm_case_initial_moments",
+             "lbm_case_function_objects", "lbm_lbmbin_write", "lbm_lbmbin_header", "lbm_lbmbin_read_field"]
+
+
+def _case_lib():
+    L = lib()
+    if not getattr(L, "_case_ready", False):
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.lbm_case_last_error.restype = ctypes.c_char_p
+        L.lbm_case_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(lbm_case_info)]
+        L.lbm_case_parameter.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.lbm_case_initial_moments.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(fp)]
+        L.lbm_case_function_objects.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint)]
+        L.lbm_lbmbin_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(fp), ctypes.c_int64, ctypes.c_int]
+        L.lbm_lbmbin_header.argtypes = [ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int)] * 5 + [
+            ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p, ctypes.c_int]
+        L.lbm_lbmbin_read_field.argtypes = [ctypes.c_char_p, ctypes.c_char_p, fp]
+        L._case_ready = True
+    return L
+
+
+def _case_check(rc):
+    if rc != 0:
+        raise LBMError(_case_lib().lbm_case_last_error().decode())
+
+
+def read_case(directory="."):
+    """programControl + latticeMesh values and the latest saved time of a case."""
+    info = lbm_case_info()
+    _case_check(_case_lib().lbm_case_read(str(directory).encode(), ctypes.byref(info)))
+    d = {name: getattr(info, name) for name, _ in lbm_case_info._fields_}
+    d["case_name"] = info.case_name.decode()
+    return d
+
+
+def case_parameter(path, name):
+    buf = ctypes.create_string_buffer(512)
+    _case_check(_case_lib().lbm_case_parameter(str(path).encode(), name.encode(), buf, 512))
+    return buf.value.decode()
+
+
+def initial_moments(directory, nx, ny, nz):
+    arrs = [np.empty(nx * ny * nz, dtype=np.float32) for _ in range(10)]
+    _case_check(_case_lib().lbm_case_initial_moments(str(directory).encode(), nx, ny, nz,
+                                                     _ptr_array(arrs, ctypes.c_float)))
+    return arrs
+
+
+def function_objects(directory):
+    m = ctypes.c_uint()
+    _case_check(_case_lib().lbm_case_function_objects(str(directory).encode(), ctypes.byref(m)))
+    return m.value
+
+
+def write_lbmbin(path, nx, ny, nz, names, fields, time_step, time_average=False):
+    arrs = [np.ascontiguousarray(f, dtype=np.float32).reshape(-1) for f in fields]
+    cn = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+    _case_check(_case_lib().lbm_lbmbin_write(str(path).encode(), nx, ny, nz, len(names), cn,
+                                             _ptr_array(arrs, ctypes.c_float), int(time_step), int(time_average)))
+
+
+def lbmbin_header(path):
+    v = [ctypes.c_int() for _ in range(5)]
+    t = ctypes.c_int64()
+    names = ctypes.create_string_buffer(4096)
+    _case_check(_case_lib().lbm_lbmbin_header(str(path).encode(), *[ctypes.byref(x) for x in v], ctypes.byref(t),
+                                              names, 4096))
+    nv = v[3].value
+    return {"nx": v[0].value, "ny": v[1].value, "nz": v[2].value, "nvars": nv, "scalar_bytes": v[4].value,
+            "time_step": t.value, "names": names.raw.split(b"\0")[:nv] and [s.decode() for s in names.raw.split(b"\0")[:nv]]}
+
+
+def read_lbmbin_field(path, name):
+    h = lbmbin_header(path)
+    out = np.empty(h["nx"] * h["ny"] * h["nz"], dtype=np.float32)
+    _case_check(_case_lib().lbm_lbmbin_read_field(str(path).encode(), name.encode(),
+                                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out

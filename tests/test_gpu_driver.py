@@ -1,0 +1,114 @@
+"""The momentBasedD3Q19/27 drivers in a case directory (GPU).
+
+Case: the reference test's case (unitTests/ComputeSanitizerTest) resized to 32^3
+with nTimeSteps 61 and saveInterval 30.  Checks: output files and their time
+stamps, fields at t=60 against the FP64 oracle started from the same initial
+conditions (velocity rel-L2 <= 1e-5), a stop/restart at t=30 reproducing the
+uninterrupted run to FP32 round-off (max |diff| <= 1e-6), and function-object
+files.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASE = os.path.join(HERE, "golden", "cases", "ComputeSanitizerTest")
+BIN = os.path.join(os.path.dirname(HERE), "cudalbm_amd", "bin")
+
+
+def make_case(d, n=32, nt=61, save=30, fo=None):
+    shutil.copytree(CASE, d, copy_function=shutil.copyfile)  # writable copies
+    def sub(f, key, val):
+        p = os.path.join(d, f)
+        lines = open(p).read().split("\n")
+        for i, l in enumerate(lines):
+            if l.startswith(key + " ") or l.startswith(key + "\t"):
+                lines[i] = f"{key}              {val};"
+                break
+        open(p, "w").write("\n".join(lines))
+    for k in ("nx", "ny", "nz"):
+        sub("latticeMesh", k, n)
+    sub("programControl", "nTimeSteps", nt)
+    sub("programControl", "saveInterval", save)
+    sub("programControl", "infoInterval", save)
+    sub("programControl", "Re", 400)
+    if fo:
+        open(os.path.join(d, "functionObjects"), "w").write("functionObjectList\n{\n" + "".join(f"    {x};\n" for x in fo) + "};\n")
+
+
+def run(exe, d):
+    r = subprocess.run([os.path.join(BIN, exe), "-GPU", "0"], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def fields(path):
+    from cudalbm_amd import native as N
+    return [N.read_lbmbin_field(path, nm) for nm in ("rho", "u", "v", "w", "m_xx", "m_xy", "m_xz", "m_yy", "m_yz", "m_zz")]
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_driver_outputs_and_parity(tmp_path, native_lib, Q):
+    from cudalbm_amd import native as N
+
+    d = str(tmp_path / "case")
+    make_case(d)
+    out = run(f"momentBasedD3Q{Q}", d)
+    assert "Time: 0" in out and "Time: 30" in out and "Time: 60" in out
+    for t in (0, 30, 60):
+        assert os.path.exists(os.path.join(d, f"lidDrivenCavity_{t}.LBMBin"))
+    h = N.lbmbin_header(os.path.join(d, "lidDrivenCavity_60.LBMBin"))
+    assert h["time_step"] == 60 and h["nvars"] == 10
+    n = 32
+    init = [a.astype(np.float64) for a in N.initial_moments(d, n, n, n)]
+    ref = O.run(Q, n, n, n, O.omega(0.0256, 400.0, n), 0.0256, init, 60)
+    got = fields(os.path.join(d, "lidDrivenCavity_60.LBMBin"))
+    vr, vg = np.stack(ref[1:4]), np.stack(got[1:4]).astype(np.float64)
+    err = np.linalg.norm(vg - vr) / np.linalg.norm(vr)
+    print(f"driver D3Q{Q} velocity rel-L2 {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_driver_restart(tmp_path, native_lib):
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    make_case(a, nt=61)
+    run("momentBasedD3Q19", a)
+    make_case(b, nt=31)
+    run("momentBasedD3Q19", b)
+    assert os.path.exists(os.path.join(b, "lidDrivenCavity_30.LBMBin"))
+    assert not os.path.exists(os.path.join(b, "lidDrivenCavity_60.LBMBin"))
+    p = os.path.join(b, "programControl")
+    text = open(p).read()
+    open(p, "w").write(text.replace("nTimeSteps              31;", "nTimeSteps              61;"))
+    out = run("momentBasedD3Q19", b)
+    assert "Restarting from lidDrivenCavity_30.LBMBin" in out
+    fa = fields(os.path.join(a, "lidDrivenCavity_60.LBMBin"))
+    fb = fields(os.path.join(b, "lidDrivenCavity_60.LBMBin"))
+    for k in range(10):
+        assert np.abs(fa[k] - fb[k]).max() <= 1e-6, k
+
+
+def test_driver_function_objects(tmp_path, native_lib):
+    from cudalbm_amd import native as N
+
+    d = str(tmp_path / "fo")
+    make_case(d, fo=["k", "kMean", "S", "SMean"])
+    run("momentBasedD3Q19", d)
+    for base, names in (("k", ["k"]), ("kMean", ["kMean"]), ("S", ["S_xx", "S_xy", "S_xz", "S_yy", "S_yz", "S_zz"]),
+                        ("SMean", ["S_xxMean", "S_xyMean", "S_xzMean", "S_yyMean", "S_yzMean", "S_zzMean"])):
+        h = N.lbmbin_header(os.path.join(d, f"{base}_60.LBMBin"))
+        assert h["names"] == names
+    # k saved at t=60 is the k of the state saved at t=60 (computed in step 59)
+    m = fields(os.path.join(d, "lidDrivenCavity_60.LBMBin"))
+    k = N.read_lbmbin_field(os.path.join(d, "k_60.LBMBin"), "k")
+    kr = 0.5 * np.sqrt(m[1].astype(np.float64) ** 2 + m[2] ** 2 + m[3] ** 2)
+    assert np.allclose(k, kr, rtol=1e-6, atol=1e-9)
+    km = N.read_lbmbin_field(os.path.join(d, "kMean_60.LBMBin"), "kMean")
+    assert np.abs(km).max() > 0 and np.all(np.abs(km) <= np.abs(k).max() * 10)
