@@ -3,16 +3,23 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Workload: lid-driven cavity, D3Q19, FP32 storage and arithmetic, 256^3 nodes,
-Re 1000, lid speed 0.05, started from rest.  One "step" is one full LBM update
-of every node (one launch of the fused step kernel); the timed region holds K
-steps on the solver's HIP stream, bracketed by hipEvents, with all state
-resident in HBM.  Prints ONE JSON line.
+Re 1000, lid speed 0.05, started from rest, with the function objects k, kMean,
+S and SMean on.  One "step" is one full LBM update of every node together with
+that step's function-object work (one launch of the fused step kernel, which
+updates the FP64 time means, plus the one-thread launch that advances the step
+counter the means divide by); the timed region holds K steps on the solver's
+HIP stream, bracketed by hipEvents, with all state resident in HBM.  Prints ONE
+JSON line.
 
-Roofline basis: algorithmic bytes per node update = the 10 stored moments read
-once and written once (80 B in FP32, 40 B in FP16); achieved GB/s = bytes per
-launch / average launch duration.  The CPU baseline is the FP64 oracle
-(oracle/, a scalar C restatement of the reference step) timed on this host on a
-bounded sample of the same workload.
+Roofline basis: algorithmic bytes per node update = the 10 stored FP32 moments
+read once and written once (80 B) plus the 7 FP64 time means (kMean, 6 SMean
+components) read once and written once (112 B): 192 B.  The instantaneous k
+and S depend only on the moments of the step they are read at, and are formed
+from them when read.  Achieved GB/s = bytes per step / average step duration.
+Variants report the plain step (80 B/node FP32, 40 B/node FP16), D3Q27 and the
+batched multi-case sweep.  The CPU baseline is the FP64 oracle (oracle/, a
+scalar C restatement of the reference step) timed on this host on a bounded
+sample of the same case.
 """
 from __future__ import annotations
 
@@ -42,16 +49,52 @@ def measure(n, Q, storage, steps, warmup, fo_mask=0, Re=1000.0, u=0.05, tile=(0,
         mom = s.get_moments() if steps + warmup >= 1 else None
     nodes = n ** 3
     mlups = nodes * steps / (ms * 1e-3) / 1e6
-    bpn = 40 if storage == "fp16" else 80
+    bpn = (40 if storage == "fp16" else 80) + (112 if fo_mask & 10 else 0)
     gbs = nodes * bpn / (ms * 1e-3 / steps) / 1e9
     finite = bool(mom is not None and all(abs(float(m.max())) < 1e3 for m in mom))
     return {"mlups": mlups, "ms_per_step": ms / steps, "achieved_gbs": gbs, "bytes_per_node": bpn,
             "tiling": list(tiling), "finite": finite}
 
 
-def cpu_baseline(target_s=10.0):
+def batch_sweep(steps, n_cases=64):
+    """Aggregate MLUPS of the seeded uneven sweep (D3Q19 FP32 cavities, N in
+    {48..160}, Re log-uniform [100, 3200], lid in [0.02, 0.08]): every case in
+    one launch per step (autotuned tiling) versus the same cases one after
+    another as single-case solvers with their default tiling."""
+    from cudalbm_amd import Batch, Solver
+    from cudalbm_amd.sweep import nodes, sweep_cases
+
+    cases = sweep_cases(n_cases)
+    total = nodes(cases)
+    t0 = time.perf_counter()
+    with Batch(cases) as b:
+        tune_s = time.perf_counter() - t0
+        b.step(4)
+        b.synchronize()
+        ms_b = b.step_timed(steps)
+        tiling = b.tiling
+        items = b.work_items
+    ms_s = 0.0
+    for nx, ny, nz, Re, u in cases:
+        with Solver(nx, ny, nz, Re, u) as s:
+            s.step(4)
+            s.synchronize()
+            ms_s += s.step_timed(steps)
+    mb = total * steps / (ms_b * 1e-3) / 1e6
+    ms = total * steps / (ms_s * 1e-3) / 1e6
+    return {"cases": n_cases, "nodes": total, "steps": steps, "mlups_batched": round(mb, 1),
+            "mlups_sequential": round(ms, 1), "speedup": round(mb / ms, 3),
+            "ms_per_step_batched": round(ms_b / steps, 4), "ms_per_step_sequential": round(ms_s / steps, 4),
+            "tiling": list(tiling[:3]), "persistent": bool(tiling[3]), "work_items": items,
+            "autotune_s": round(tune_s, 2),
+            "roofline_frac_batched": round(total * 80 / (ms_b * 1e-3 / steps) / 1e9 / PEAK_HBM_GBS, 4)}
+
+
+def cpu_baseline(target_s=25.0):
     """FP64 oracle, D3Q19 96^3 cavity (Re 1000, lid 0.05) from rest, on this
-    host's cores; the step count is sized to about target_s seconds."""
+    host's cores; the step count is sized to about target_s seconds.  The
+    oracle step carries no function-object work, so the CPU figure is an upper
+    bound for the CPU on the headline workload."""
     from oracle import oracle as O
 
     n = 96
@@ -78,7 +121,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="only the headline D3Q19 FP32 step")
-    ap.add_argument("--only", default="", help="profile one variant: q19fp32|q19fp16|q19fo|q27fp32|q27fp16")
+    ap.add_argument("--only", default="",
+                    help="profile one variant: q19fo|q19fp32|q19fp16|q27fp32|q27fp16|batch")
     ap.add_argument("--tile", default="0,0,0")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (x4), to calibrate HBM byte counters")
@@ -98,29 +142,36 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     tile = tuple(int(v) for v in args.tile.split(","))
+    FO_ALL = 15  # k | kMean | S | SMean
 
     if args.only:
-        spec = {"q19fp32": (19, "fp32", 0), "q19fp16": (19, "fp16", 0), "q19fo": (19, "fp32", 15),
+        if args.only == "batch":
+            print(json.dumps({"variant": "batch", **batch_sweep(args.steps)}), flush=True)
+            return
+        spec = {"q19fo": (19, "fp32", FO_ALL), "q19fp32": (19, "fp32", 0), "q19fp16": (19, "fp16", 0),
                 "q27fp32": (27, "fp32", 0), "q27fp16": (27, "fp16", 0)}[args.only]
         r = measure(args.n, spec[0], spec[1], args.steps, args.warmup, fo_mask=spec[2], tile=tile)
         print(json.dumps({"variant": args.only, **r}), flush=True)
         return
 
-    head = measure(args.n, 19, "fp32", args.steps, args.warmup, tile=tile)
-    print(f"# D3Q19 fp32 {args.n}^3: {head['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    head = measure(args.n, 19, "fp32", args.steps, args.warmup, fo_mask=FO_ALL, tile=tile)
+    print(f"# D3Q19 fp32 {args.n}^3 + k/kMean/S/SMean: {head['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     variants = {}
     if not args.no_variants:
         k = max(50, args.steps // 4)
-        for name, (Q, st, fo) in {"d3q19_fp16": (19, "fp16", 0), "d3q19_fp32_fo_k_kMean_S_SMean": (19, "fp32", 15),
-                                  "d3q27_fp32": (27, "fp32", 0), "d3q27_fp16": (27, "fp16", 0)}.items():
+        for name, (Q, st, fo) in {"d3q19_fp32_step_only": (19, "fp32", 0), "d3q19_fp16_step_only": (19, "fp16", 0),
+                                  "d3q19_fp16_fo": (19, "fp16", FO_ALL), "d3q27_fp32_step_only": (27, "fp32", 0),
+                                  "d3q27_fp16_step_only": (27, "fp16", 0)}.items():
             r = measure(args.n, Q, st, k, 10, fo_mask=fo)
             variants[name] = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
                               "roofline_frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4),
                               "bytes_per_node": r["bytes_per_node"], "tiling": r["tiling"]}
             print(f"# {name}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        variants["batch_sweep64_d3q19_fp32"] = batch_sweep(100)
+        print(f"# batch sweep: {variants['batch_sweep64_d3q19_fp32']}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
-        "metric": "MLUPS (D3Q19 256^3 lid-driven cavity, FP32)",
+        "metric": "MLUPS (D3Q19 256^3 lid-driven cavity, FP32, k/kMean/S/SMean on)",
         "value": round(head["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -132,7 +183,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: lid-driven cavity from rest (no dataset)",
-        "config": {"workload": f"lid-driven cavity D3Q19 {args.n}^3 FP32, Re 1000, lid 0.05, rest start",
+        "config": {"workload": f"lid-driven cavity D3Q19 {args.n}^3 FP32, Re 1000, lid 0.05, rest start, "
+                               "function objects k, kMean, S, SMean on",
                    "velocity_set": "D3Q19", "nodes": args.n ** 3, "tiling": head["tiling"]},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,

@@ -4,6 +4,6 @@ The compute path is the HIP library liblbm_amd.so (C ABI: include/lbm_amd.h);
 this package holds its sources (csrc/), the build recipe and the host-side
 binding that mirrors the reference's case interface.
 """
-from .native import Solver, LBMError  # noqa: F401
+from .native import Batch, LBMError, Solver, Surrogate, autotune  # noqa: F401
 
-__all__ = ["Solver", "LBMError"]
+__all__ = ["Solver", "Batch", "Surrogate", "autotune", "LBMError"]

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblbm_amd.so")
 
-HIP_SOURCES = ["lbm_capi.hip", "lbm_batch.hip"]
+HIP_SOURCES = ["lbm_capi.hip", "lbm_batch.hip", "surrogate.hip"]
 HOST_SOURCES = ["case_io.cpp"]
 
 HIPCC_FLAGS = [

@@ -7,29 +7,46 @@
 namespace lbm {
 
 // FP32 staging <-> FP16 encoded moments.
-__global__ void encode_f16_kernel(const float *__restrict__ src, _Float16 *__restrict__ dst, int64_t n,
+static __global__ void encode_f16_kernel(const float *__restrict__ src, _Float16 *__restrict__ dst, int64_t n,
                                   float enc) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = _Float16(src[i] * enc);
 }
-__global__ void decode_f16_kernel(const _Float16 *__restrict__ src, float *__restrict__ dst, int64_t n,
+static __global__ void decode_f16_kernel(const _Float16 *__restrict__ src, float *__restrict__ dst, int64_t n,
                                   float dec) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = float(src[i]) * dec;
 }
 
 // Advances the absolute step counter read by the fused time averages.
-__global__ void tick_kernel(int64_t *t) {
+static __global__ void tick_kernel(int64_t *t) {
     if (threadIdx.x == 0 && blockIdx.x == 0) t[0] += 1;
 }
 
-__global__ void fill_kernel_f64(double *p, int64_t n, double v) {
+static __global__ void fill_kernel_f64(double *p, int64_t n, double v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
-__global__ void fill_kernel_f32(float *p, int64_t n, float v) {
+static __global__ void fill_kernel_f32(float *p, int64_t n, float v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// Instantaneous k and S (either output may be null) from the stored moments.
+template <typename ST>
+__global__ void fo_instant_kernel(const StepParams p, float *__restrict__ k, float *__restrict__ S) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < p.n; i += (int64_t)gridDim.x * blockDim.x) {
+        float m[10];
+#pragma unroll
+        for (int c = 0; c < 10; c++) m[c] = Store<ST>::load(p.mom[c], (int)i, p.dec[c]);
+        float kk, s[6];
+        fo_values(m, p.tau, kk, s);
+        if (k) k[i] = kk;
+        if (S) {
+#pragma unroll
+            for (int a = 0; a < 6; a++) S[a * p.n + i] = s[a];
+        }
+    }
 }
 
 // Wave64 sum by butterfly shuffles, then across the waves of the block via LDS.
@@ -50,12 +67,12 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 
 // Per-block partial sums of kinetic energy, |u - u_prev|^2 and |u|^2 (physical u),
 // then u_prev <- u.  Blocks stride over the node range in a fixed pattern.
+// Block j of nb covers nodes j*NT + tid + k*nb*NT; part receives its 3 sums.
 template <typename ST, int NT>
-__global__ __launch_bounds__(NT) void residual_partials_kernel(const StepParams p, float *__restrict__ uprev,
-                                                               double *__restrict__ part, int first) {
-    __shared__ double sh[NT / 64];
+__device__ __forceinline__ void residual_partials(const StepParams &p, float *__restrict__ uprev,
+                                                  double *__restrict__ part, int first, int j, int nb, double *sh) {
     double ke = 0.0, du = 0.0, uu = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < p.n; i += (int64_t)gridDim.x * NT) {
+    for (int64_t i = j * (int64_t)NT + threadIdx.x; i < p.n; i += (int64_t)nb * NT) {
         // physical rho and u formed in FP64 from the stored values
         const double rho = 1.0 + (double)Store<ST>::load(p.mom[0], (int)i, p.dec[0]);
         double u[3];
@@ -77,17 +94,27 @@ __global__ __launch_bounds__(NT) void residual_partials_kernel(const StepParams 
     du = block_sum<NT>(du, sh);
     uu = block_sum<NT>(uu, sh);
     if (threadIdx.x == 0) {
-        part[3 * blockIdx.x + 0] = ke;
-        part[3 * blockIdx.x + 1] = du;
-        part[3 * blockIdx.x + 2] = uu;
+        part[0] = ke;
+        part[1] = du;
+        part[2] = uu;
     }
 }
 
-// Sums the per-block partials in block order (one block of NT threads).
+template <typename ST, int NT>
+__global__ __launch_bounds__(NT) void residual_partials_kernel(const StepParams p, float *__restrict__ uprev,
+                                                               double *__restrict__ part, int first) {
+    __shared__ double sh[NT / 64];
+    residual_partials<ST, NT>(p, uprev, part + 3 * blockIdx.x, first, blockIdx.x, gridDim.x, sh);
+}
+
+// Sums per-block partials in block order: block b of the grid reduces the
+// nblocks partials starting at part + 3*nblocks*b into out[3b..3b+2].
 template <int NT>
 __global__ __launch_bounds__(NT) void residual_final_kernel(const double *__restrict__ part, int nblocks,
                                                             double *__restrict__ out) {
     __shared__ double sh[NT / 64];
+    part += 3 * (int64_t)nblocks * blockIdx.x;
+    out += 3 * blockIdx.x;
     double a[3] = {0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < nblocks; i += NT)
         for (int k = 0; k < 3; k++) a[k] += part[3 * i + k];

@@ -22,22 +22,6 @@ namespace {
 
 thread_local std::string g_create_error;
 
-struct HipError {
-    hipError_t e;
-    const char *what;
-};
-
-#define HIPX(expr)                                               \
-    do {                                                         \
-        hipError_t e_ = (expr);                                  \
-        if (e_ != hipSuccess) throw HipError{e_, #expr};         \
-    } while (0)
-
-struct ArgError {
-    int code;
-    std::string msg;
-};
-
 // ------------------------------------------------------------ dispatch
 template <int Q, int TX, int TY>
 void launch_step_t(const StepParams &p, int grid, hipStream_t s, int storage, bool fo, bool probe) {
@@ -197,7 +181,7 @@ struct lbm_handle {
     double *part = nullptr, *red_out = nullptr;
     int red_blocks = 1024;
     bool red_first = true;
-    std::map<int, hipGraphExec_t> graphs;  // key: parity*2 + fo flag, for kGraphSteps steps
+    std::map<int, hipGraphExec_t> graphs;  // key: parity*2 + means flag, for kGraphSteps steps
     std::map<int, std::vector<int64_t>> perms;
     std::string err;
 
@@ -208,7 +192,8 @@ struct lbm_handle {
     }
 
     StepParams params(bool probe = false) const {
-        StepParams p{};
+        StepParams p;
+        std::memset(&p, 0, sizeof p);
         p.nx = cfg.nx;
         p.ny = cfg.ny;
         p.nz = cfg.nz;
@@ -228,9 +213,7 @@ struct lbm_handle {
         p.zoff = zoff;
         codec(p.enc, p.dec);
         p.fo_mask = probe ? 0u : fo_mask;
-        p.k = k;
         p.kmean = kmean;
-        p.S = S;
         p.smean = smean;
         p.tcount = tcount;
         p.n = n;
@@ -247,6 +230,8 @@ struct lbm_handle {
         }
     }
     int grid() const { return nbx * nby * nc; }
+    // the step carries the FP64 time means (instantaneous k/S are formed on read)
+    bool means() const { return (fo_mask & (LBM_FO_KMEAN | LBM_FO_SMEAN)) != 0; }
     size_t elem() const { return cfg.storage == LBM_STORE_FP16 ? 2 : 4; }
 
     void drop_graphs() {
@@ -275,6 +260,12 @@ struct lbm_handle {
         stream = nullptr;
     }
 };
+
+void lbm::handle_dims(const lbm_handle *h, int *nx, int *ny, int *nz) {
+    *nx = h->cfg.nx;
+    *ny = h->cfg.ny;
+    *nz = h->cfg.nz;
+}
 
 namespace {
 
@@ -371,16 +362,15 @@ void enqueue_step(lbm_handle *h, int parity) {
     StepParams p = h->params();
     p.fin = h->faces[parity];
     p.fout = h->faces[parity ^ 1];
-    const bool fo = h->fo_mask != 0;
-    launch_step(h->Q, h->TX, h->TY, p, h->grid(), h->stream, h->cfg.storage, fo, false);
-    if (h->fo_mask & (LBM_FO_KMEAN | LBM_FO_SMEAN)) {
+    launch_step(h->Q, h->TX, h->TY, p, h->grid(), h->stream, h->cfg.storage, h->means(), false);
+    if (h->means()) {
         hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, h->stream, h->tcount);
         HIPX(hipGetLastError());
     }
 }
 
 hipGraphExec_t graph_for(lbm_handle *h) {
-    const int key = h->parity * 2 + (h->fo_mask ? 1 : 0);
+    const int key = h->parity * 2 + (h->means() ? 1 : 0);
     auto it = h->graphs.find(key);
     if (it != h->graphs.end()) return it->second;
     hipGraph_t g = nullptr;
@@ -461,10 +451,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     h->nc = c.nz / h->LZ;
     h->n = (int64_t)c.nx * c.ny * c.nz;
     if (h->n >= (int64_t)INT32_MAX) return bad(LBM_ERR_SIZE, "mesh too large: node indices are 32-bit");
-    // latticeMesh.cuh:178-182 (computed in FP64 here)
-    const double nu = c.u_inf * (double)(c.nx - 1) / c.Re;
-    h->tau = 0.5 + 3.0 * nu;
-    h->omega = 1.0 / h->tau;
+    relaxation(c.Re, c.u_inf, c.nx, &h->omega, &h->tau);
     try {
         HIPX(hipSetDevice(c.device));
         if (c.stream)
@@ -645,15 +632,6 @@ int lbm_fo_enable(lbm_handle *h, unsigned mask) {
         if ((mask & LBM_FO_S) && !h->S) HIPX(hipMalloc(&h->S, sizeof(float) * 6 * n));
         if ((mask & LBM_FO_KMEAN) && !h->kmean) HIPX(hipMalloc(&h->kmean, sizeof(double) * n));
         if ((mask & LBM_FO_SMEAN) && !h->smean) HIPX(hipMalloc(&h->smean, sizeof(double) * 6 * n));
-        // instantaneous fields start at zero too (objectAllocator allocates them zeroed)
-        if (mask & LBM_FO_K) {
-            hipLaunchKernelGGL(fill_kernel_f32, dim3(2048), dim3(256), 0, h->stream, h->k, n, 0.0f);
-            HIPX(hipGetLastError());
-        }
-        if (mask & LBM_FO_S) {
-            hipLaunchKernelGGL(fill_kernel_f32, dim3(2048), dim3(256), 0, h->stream, h->S, 6 * n, 0.0f);
-            HIPX(hipGetLastError());
-        }
         if (mask & LBM_FO_KMEAN) {
             hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, h->kmean, n, 0.0);
             HIPX(hipGetLastError());
@@ -684,6 +662,15 @@ int lbm_fo_get(lbm_handle *h, int field, double *out, int layout) {
         const auto &perm = h->perm(layout);
         std::vector<double> tmp((size_t)(n * ncomp));
         if (field == LBM_FIELD_K || field == LBM_FIELD_S) {
+            // instantaneous fields from the current stored moments
+            float *dk = field == LBM_FIELD_K ? h->k : nullptr, *dS = field == LBM_FIELD_S ? h->S : nullptr;
+            const StepParams p = h->params();
+            if (h->cfg.storage == LBM_STORE_FP32)
+                hipLaunchKernelGGL(fo_instant_kernel<float>, dim3(2048), dim3(256), 0, h->stream, p, dk, dS);
+            else
+                hipLaunchKernelGGL(fo_instant_kernel<_Float16>, dim3(2048), dim3(256), 0, h->stream, p, dk, dS);
+            HIPX(hipGetLastError());
+            HIPX(hipStreamSynchronize(h->stream));
             std::vector<float> f((size_t)(n * ncomp));
             HIPX(hipMemcpy(f.data(), field == LBM_FIELD_K ? h->k : h->S, sizeof(float) * n * ncomp,
                            hipMemcpyDeviceToHost));

@@ -4,11 +4,36 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "step_kernel.hpp"
 
 namespace lbm {
+
+// Failures inside the library are C++ exceptions, turned into status codes at
+// the C ABI (no exception crosses it).
+struct HipError {
+    hipError_t e;
+    const char *what;
+};
+struct ArgError {
+    int code;
+    std::string msg;
+};
+#define HIPX(expr)                                       \
+    do {                                                 \
+        hipError_t e_ = (expr);                          \
+        if (e_ != hipSuccess) throw ::lbm::HipError{e_, #expr}; \
+    } while (0)
+
+// Relaxation of a cavity case, latticeMesh.cuh:178-182 evaluated in FP64:
+// nu = u (nx-1)/Re, tau = 0.5 + 3 nu, omega = 1/tau.
+inline void relaxation(double Re, double u_inf, int nx, double *omega, double *tau) {
+    const double nu = u_inf * (double)(nx - 1) / Re;
+    *tau = 0.5 + 3.0 * nu;
+    *omega = 1.0 / *tau;
+}
 
 bool tile_supported(int tx, int ty);
 void launch_step(int Q, int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool fo,
@@ -19,5 +44,12 @@ int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_
 int choose_lz(int nx, int ny, int nz, int tx, int ty);
 void choose_tile(int Q, int nx, int ny, int *tx, int *ty);
 std::vector<int64_t> make_perm(int layout, int nx, int ny, int nz, int tx, int ty);
+
+}  // namespace lbm
+
+struct lbm_handle;
+namespace lbm {
+// Mesh size of a solver handle (library-internal accessor).
+void handle_dims(const lbm_handle *h, int *nx, int *ny, int *nz);
 
 }  // namespace lbm

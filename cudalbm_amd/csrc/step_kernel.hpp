@@ -44,11 +44,9 @@ struct StepParams {
     float *fout;           // face buffers written (this step)
     int yoff, zoff;        // offsets of the Y and Z face blocks inside one face set
     float enc[10], dec[10];  // FP16 storage codec: stored16 = m*enc, m = stored16*dec
-    // fused function objects
+    // fused FP64 time means of the function objects (fo_mask bits 2: kMean, 8: SMean)
     unsigned fo_mask;
-    float *k;
     double *kmean;
-    float *S;              // 6 component arrays, each of n floats
     double *smean;         // 6 component arrays, each of n doubles
     const int64_t *tcount; // absolute step index of this step (device counter)
     int64_t n;             // nodes
@@ -364,239 +362,74 @@ __device__ __forceinline__ void load_next(const StepParams &p, const float *__re
     }
 }
 
-// The fused step.  PROBE replaces the moment algebra by integer ids so that
-// tests can read back the exact source of every population.
+// Function-object values of one node from its stored (scaled) moments:
+// k = kineticEnergy.cuh:71-85, S = strainRateTensor.cuh:71-84 (tau = 1/omega).
+__device__ __forceinline__ void fo_values(const float m[10], float tau, float &kk, float S[6]) {
+    kk = 0.5f * sqrtf(m[1] * m[1] + m[2] * m[2] + m[3] * m[3]);
+    const float it = 1.0f / tau;
+    S[0] = 3.f * (m[1] * m[1] - m[4]) / 9.f * it;
+    S[1] = 3.f * (m[1] * m[2] - m[5]) / 18.f * it;
+    S[2] = 3.f * (m[1] * m[3] - m[6]) / 18.f * it;
+    S[3] = 3.f * (m[2] * m[2] - m[7]) / 9.f * it;
+    S[4] = 3.f * (m[2] * m[3] - m[8]) / 18.f * it;
+    S[5] = 3.f * (m[3] * m[3] - m[9]) / 9.f * it;
+}
+
+// The fused step (body in step_body.inc).  PROBE replaces the moment algebra by
+// integer ids so that tests can read back the exact source of every population.
 //
-// LDS: NS population slots of (TX+2)x(TY+2) (in-plane and down-going pops of
-// this plane, up-going pops of the previous plane in bank 0), NU more slots for
-// the second bank of up-going pops (the bank alternates with the plane parity so
-// plane z can stage its own up-going pops while plane z-1's are being read), and
-// two parity buffers of the edge nodes' moments for the deferred face writes.
+// One cavity: workgroup b owns column (bx, by) and chunk c of the domain.  The
+// parameters are the kernel argument itself.
 template <int Q, int TX, int TY, typename ST, bool FO, bool PROBE>
 __global__ __launch_bounds__(TX *TY, Q == 19 ? 4 : 3) void step_kernel(const StepParams p) {
-    using G = Geo<TX, TY>;
-    using R = Ring<Q, TX, TY>;
-    constexpr int QF = VS<Q>::QF;
-    constexpr int NS = VS<Q>::NSLOT;
-    constexpr int NU = VS<Q>::NU;
-    using E = Edge<TX, TY>;
-    __shared__ float lds[(NS + NU) * G::P + 2 * 10 * E::NE];
-    float *const edge_m = lds + (NS + NU) * G::P;  // [plane parity][10][NE]
-    const int my_edge = E::index(threadIdx.x % TX, threadIdx.x / TX);
-
     const int tid = threadIdx.x;
-    const int tx = tid % TX, ty = tid / TX;
-    const int b = blockIdx.x;
-    const int bx = b % p.nbx;
-    const int by = (b / p.nbx) % p.nby;
+#define LBM_WORK_ITEM                      \
+    const int b = blockIdx.x;              \
+    const int bx = b % p.nbx;              \
+    const int by = (b / p.nbx) % p.nby;    \
     const int c = b / (p.nbx * p.nby);
-    const int x = bx * TX + tx, y = by * TY + ty;
-    const int z0 = c * p.lz, z1 = z0 + p.lz - 1;
-    const float *__restrict__ fin = p.fin;
-    float *__restrict__ fout = p.fout;
-    const int own = (ty + 1) * G::WP + (tx + 1);
+#include "step_body.inc"
+#undef LBM_WORK_ITEM
+}
 
-    Job jobs[R::JPT];
-#pragma unroll
-    for (int i = 0; i < R::JPT; i++) jobs[i] = make_job<Q, TX, TY>(tid + i * G::NT, p, bx, by);
-
-    // Up-going populations of plane z0-1 at this column (top face of the chunk
-    // below; zero under the domain) into the bank plane z0 reads.
-    float p5;
-    {
-        const int bank = (z0 - 1) & 1;
-        unroll<1, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            if constexpr (czq(q) == 1) {
-                constexpr int k = face_k<Q>(2, q);
-                const float v = c > 0 ? fin[zf_off<Q>(p, 1, c - 1, k, y, x)] : 0.f;
-                if constexpr (q == 5)
-                    p5 = v;
-                else
-                    lds[u_addr<Q, TX, TY>(q, bank) + own] = v;
-            }
-        });
-    }
-    // mc: old moments of plane z; mn: of plane z+1, or -- on the chunk's last
-    // plane -- the down-going populations of plane z1+1 (bottom face of the chunk
-    // above, zero above the domain), fetched into the idle prefetch slot.
-    float mc[10], mn[10];
-    if constexpr (!PROBE) {
-        load_plane<ST, TX, TY>(p, bx, by, z0, tid, mc);
+// Many cavities in one launch: work item i of a tile-to-case table names a case
+// and one (column, chunk) of it; the case's parameters and buffers come from a
+// per-case table in device memory (read with scalar loads).  One launch
+// advances every listed case by one step (FP32 storage, no function objects).
+// PERSIST = false: one workgroup per item.  PERSIST = true: a persistent grid
+// whose workgroups stride over the items; it is built for one workgroup fewer
+// per CU, because the item loop needs more registers than a single pass.
+struct BatchItem {
+    int cs, bx, by, c;
+};
+template <int Q, int TX, int TY, bool PERSIST>
+__global__ __launch_bounds__(TX *TY, PERSIST ? (Q == 19 ? 3 : 2) : (Q == 19 ? 4 : 3)) void batch_step_kernel(
+    const StepParams *__restrict__ cases, const BatchItem *__restrict__ items, int count) {
+    using ST = float;
+    constexpr bool FO = false, PROBE = false;
+    if constexpr (!PERSIST) {
+        if ((int)blockIdx.x >= count) return;
+        const BatchItem it = items[blockIdx.x];
+        const StepParams &p = cases[it.cs];
+        const int tid = threadIdx.x;
+#define LBM_WORK_ITEM const int bx = it.bx, by = it.by, c = it.c;
+#include "step_body.inc"
     } else {
-#pragma unroll
-        for (int k = 0; k < 10; k++) mc[k] = 0.f;
-    }
-    load_next<Q, ST, TX, TY, PROBE>(p, fin, bx, by, c, z0 + 1, z1, tid, x, y, mn);
-    float ring[R::JPT];
-#pragma unroll
-    for (int i = 0; i < R::JPT; i++) ring[i] = job_raw(jobs[i], fin, z0, p.nz);
-
-    for (int z = z0; z <= z1; z++) {
-        // ---- x/y faces of plane z-2: its edge moments were parked two iterations
-        //      ago; that parity buffer is rewritten only after this plane's barriers
-        if (z - 2 >= z0) flush_xy_faces<Q, TX, TY, PROBE>(p, fout, edge_m + (z & 1) * 10 * E::NE, tid, bx, by, z - 2);
-        // ---- prefetch plane z+2 and the ring words of plane z+1 (unconditional
-        //      loads of clamped addresses; unused values are discarded)
-        float mq[10];
-        if (z + 2 <= z1 + 1) load_next<Q, ST, TX, TY, PROBE>(p, fin, bx, by, c, z + 2, z1, tid, x, y, mq);
-        float ring_n[R::JPT];
-#pragma unroll
-        for (int i = 0; i < R::JPT; i++) ring_n[i] = job_raw(jobs[i], fin, z + 1, p.nz);
-
-        const int bank_rd = (z - 1) & 1, bank_wr = z & 1;
-        // ---- own populations into LDS
-        float g[27];
-        float p5_next;
-        {
-            const Rec r = rec_setup(mc);
-            g[0] = pop_of<Q, 0, PROBE>(r, mc, x, y, z, p);
-            unroll<1, Q>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                if constexpr (czq(q) == 0) {
-                    lds[lds_slot(q) * G::P + own] = pop_of<Q, q, PROBE>(r, mc, x, y, z, p);
-                } else if constexpr (czq(q) == 1) {
-                    const float v = pop_of<Q, q, PROBE>(r, mc, x, y, z, p);
-                    if constexpr (q == 5)
-                        p5_next = v;
-                    else
-                        lds[u_addr<Q, TX, TY>(q, bank_wr) + own] = v;
-                }
-            });
-        }
-        g[5] = p5;
-        if (z < z1) {
-            const Rec r = rec_setup(mn);
-            unroll<1, Q>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                if constexpr (czq(q) == -1) {
-                    const float v = pop_of<Q, q, PROBE>(r, mn, x, y, z + 1, p);
-                    if constexpr (q == 6)
-                        g[6] = v;
-                    else
-                        lds[lds_slot(q) * G::P + own] = v;
-                }
-            });
-        } else {
-            unroll<1, Q>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                if constexpr (czq(q) == -1) {
-                    const float v = mn[face_k<Q>(2, q)];
-                    if constexpr (q == 6)
-                        g[6] = v;
-                    else
-                        lds[lds_slot(q) * G::P + own] = v;
-                }
-            });
-        }
-#pragma unroll
-        for (int i = 0; i < R::JPT; i++)
-            if (jobs[i].lds >= 0)
-                lds[bank_rd ? jobs[i].lds1() : jobs[i].lds0()] = job_value(jobs[i], ring[i], z, p.nz);
-
-        lds_barrier();
-        unroll<1, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int off = -cyq(q) * G::WP - cxq(q);
-            if constexpr (czq(q) == 1 && q != 5)
-                g[q] = lds[u_addr<Q, TX, TY>(q, bank_rd) + own + off];
-            else if constexpr (q != 5 && q != 6)
-                g[q] = lds[lds_slot(q) * G::P + own + off];
-        });
-        lds_barrier();
-
-        const int bits = node_bits(x, y, z, p.nx, p.ny, p.nz);
-        if constexpr (PROBE) {
-            const int d = x + p.nx * (y + p.ny * z);
-#pragma unroll
-            for (int q = 0; q < Q; q++) p.probe_src[(int64_t)d * Q + q] = __float_as_uint(g[q]);
-            p.probe_type[d] = (uint8_t)bits;
-            float mz[10];
-#pragma unroll
-            for (int k = 0; k < 10; k++) mz[k] = 0.f;
-
-        } else {
-            const Sums s = pop_sums<Q>(g);
-            float m[10];
-            if (bits & 0x40)
-                boundary_moments<Q>(s, bits, p.omega, p.u_lid, m);
-            else
-                interior_moments<Q>(s, m);
-            collide(m, p.t_omega, p.omega_h, p.omega);
-            const unsigned pb = plane_base<TX, TY>(p, bx, by, z);
-            const unsigned i = pb + (unsigned)tid;
-#pragma unroll
-            for (int k = 0; k < 10; k++) Store<ST>::store(p.mom[k], (int)i, m[k], p.enc[k]);
-            if (my_edge >= 0) {
-                // neighbours must see the same (rounded) moments the next step reloads
-                float *em = edge_m + (z & 1) * 10 * E::NE;
-#pragma unroll
-                for (int k = 0; k < 10; k++)
-                    em[k * E::NE + my_edge] = sizeof(ST) < 4 ? float(ST(m[k] * p.enc[k])) * p.dec[k] : m[k];
+        for (int i = blockIdx.x; i < count; i += gridDim.x) {
+            const BatchItem it = items[i];
+            const StepParams &p = cases[it.cs];
+            // an opaque copy of the thread index per item keeps per-thread
+            // addressing from being hoisted out of the item loop (it would stay
+            // live through the whole body and spill)
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            {
+#include "step_body.inc"
             }
-            if constexpr (FO) {
-                // kineticEnergy.cuh:71-85 and strainRateTensor.cuh:71-84 on the
-                // stored (scaled) post-collision moments; FP64 running means.
-                const float kk = 0.5f * sqrtf(m[1] * m[1] + m[2] * m[2] + m[3] * m[3]);
-                const float it = 1.0f / p.tau;
-                float S[6];
-                S[0] = 3.f * (m[1] * m[1] - m[4]) / 9.f * it;
-                S[1] = 3.f * (m[1] * m[2] - m[5]) / 18.f * it;
-                S[2] = 3.f * (m[1] * m[3] - m[6]) / 18.f * it;
-                S[3] = 3.f * (m[2] * m[2] - m[7]) / 9.f * it;
-                S[4] = 3.f * (m[2] * m[3] - m[8]) / 18.f * it;
-                S[5] = 3.f * (m[3] * m[3] - m[9]) / 9.f * it;
-                if (p.fo_mask & 1u) p.k[i] = kk;
-                if (p.fo_mask & 4u) {
-#pragma unroll
-                    for (int a = 0; a < 6; a++) p.S[a * p.n + i] = S[a];
-                }
-                if (p.fo_mask & 10u) {
-                    const double inv = 1.0 / double(p.tcount[0] + 1);
-                    if (p.fo_mask & 2u) {
-                        const double o = p.kmean[i];
-                        p.kmean[i] = o + (double(kk) - o) * inv;
-                    }
-                    if (p.fo_mask & 8u) {
-#pragma unroll
-                        for (int a = 0; a < 6; a++) {
-                            const double o = p.smean[a * p.n + i];
-                            p.smean[a * p.n + i] = o + (double(S[a]) - o) * inv;
-                        }
-                    }
-                }
-            }
+            __syncthreads();  // the next item rewrites the LDS the face flush above read
         }
-
-        // ---- rotate the plane pipeline
-        p5 = p5_next;
-#pragma unroll
-        for (int k = 0; k < 10; k++) {
-            mc[k] = mn[k];
-            mn[k] = mq[k];
-        }
-#pragma unroll
-        for (int i = 0; i < R::JPT; i++) ring[i] = ring_n[i];
     }
-    // faces of the chunk's last two planes
-    lds_barrier();
-    if (z1 - 1 >= z0) flush_xy_faces<Q, TX, TY, PROBE>(p, fout, edge_m + ((z1 - 1) & 1) * 10 * E::NE, tid, bx, by, z1 - 1);
-    flush_xy_faces<Q, TX, TY, PROBE>(p, fout, edge_m + (z1 & 1) * 10 * E::NE, tid, bx, by, z1);
-    // z faces of the chunk's end planes, from the moments this thread stored
-    // (re-read: keeping them live through the plane loop costs more registers
-    // than 2/lz of a plane's reads)
-    if (c > 0) {
-        float m[10];
-        if constexpr (!PROBE) load_plane<ST, TX, TY>(p, bx, by, z0, tid, m);
-        else for (int k = 0; k < 10; k++) m[k] = 0.f;
-        write_z_faces<Q, PROBE>(p, fout, m, x, y, z0, c, z0, z1 + 1);
-    }
-    if (c < p.nc - 1) {
-        float m[10];
-        if constexpr (!PROBE) load_plane<ST, TX, TY>(p, bx, by, z1, tid, m);
-        else for (int k = 0; k < 10; k++) m[k] = 0.f;
-        write_z_faces<Q, PROBE>(p, fout, m, x, y, z1, c, z0 - 1, z1);
-    }
+#undef LBM_WORK_ITEM
 }
 
 // Face buffers from the current moments (post-collision populations of every

@@ -315,3 +315,320 @@ def read_lbmbin_field(path, name):
     _case_check(_case_lib().lbm_lbmbin_read_field(str(path).encode(), name.encode(),
                                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
     return out
+
+
+# ------------------------------------------------------------------ batches
+class lbm_batch_case(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("Re", ctypes.c_double), ("u_inf", ctypes.c_double)]
+
+
+class lbm_batch_config(ctypes.Structure):
+    _fields_ = [("velocity_set", ctypes.c_int), ("device", ctypes.c_int), ("stream", ctypes.c_void_p),
+                ("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("tile_z", ctypes.c_int),
+                ("persistent", ctypes.c_int)]
+
+
+EXPORTED += ["lbm_autotune", "lbm_batch_create", "lbm_batch_create_error", "lbm_batch_destroy",
+             "lbm_batch_last_error", "lbm_batch_get_tiling", "lbm_batch_tuning_report", "lbm_batch_work_items",
+             "lbm_batch_set_active", "lbm_batch_step", "lbm_batch_step_timed", "lbm_batch_synchronize",
+             "lbm_batch_get_steps", "lbm_batch_set_moments", "lbm_batch_get_moments", "lbm_batch_reduce",
+             "lbm_batch_run"]
+
+
+def _batch_lib():
+    L = lib()
+    if not getattr(L, "_batch_ready", False):
+        B = ctypes.c_void_p
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        fpp = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
+        L.lbm_autotune.argtypes = [ctypes.POINTER(lbm_config), ctypes.c_int, ip, ip, ip, dp]
+        L.lbm_batch_create.argtypes = [ctypes.POINTER(lbm_batch_config), ctypes.POINTER(lbm_batch_case),
+                                       ctypes.c_int, ctypes.POINTER(B)]
+        L.lbm_batch_create_error.restype = ctypes.c_char_p
+        L.lbm_batch_destroy.argtypes = [B]
+        L.lbm_batch_last_error.argtypes = [B]
+        L.lbm_batch_last_error.restype = ctypes.c_char_p
+        L.lbm_batch_get_tiling.argtypes = [B, ip, ip, ip, ip]
+        L.lbm_batch_tuning_report.argtypes = [B, ctypes.c_char_p, ctypes.c_int]
+        L.lbm_batch_work_items.argtypes = [B, ip]
+        L.lbm_batch_set_active.argtypes = [B, ctypes.POINTER(ctypes.c_uint8)]
+        L.lbm_batch_step.argtypes = [B, ctypes.c_int]
+        L.lbm_batch_step_timed.argtypes = [B, ctypes.c_int, dp]
+        L.lbm_batch_synchronize.argtypes = [B]
+        L.lbm_batch_get_steps.argtypes = [B, i64p]
+        L.lbm_batch_set_moments.argtypes = [B, ctypes.c_int, fpp, ctypes.c_int]
+        L.lbm_batch_get_moments.argtypes = [B, ctypes.c_int, fpp, ctypes.c_int]
+        L.lbm_batch_reduce.argtypes = [B, dp, dp]
+        L.lbm_batch_run.argtypes = [B, ctypes.c_double, ctypes.c_int, ctypes.c_int64, i64p, dp, dp]
+        L._batch_ready = True
+    return L
+
+
+def autotune(nx, ny, nz, Re, u_inf, velocity_set=19, storage="fp32", steps=20, device=0):
+    """Fastest (tile_x, tile_y, tile_z) for one case and its ms per step."""
+    L = _batch_lib()
+    cfg = lbm_config()
+    cfg.velocity_set = velocity_set
+    cfg.storage = {"fp32": LBM_STORE_FP32, "fp16": LBM_STORE_FP16}[storage]
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.Re, cfg.u_inf = float(Re), float(u_inf)
+    cfg.device = device
+    t = [ctypes.c_int() for _ in range(3)]
+    ms = ctypes.c_double()
+    rc = L.lbm_autotune(ctypes.byref(cfg), int(steps), *[ctypes.byref(x) for x in t], ctypes.byref(ms))
+    if rc != 0:
+        raise LBMError(f"lbm_autotune failed ({rc}): {L.lbm_batch_create_error().decode()}")
+    return (t[0].value, t[1].value, t[2].value), ms.value
+
+
+class Batch:
+    """Independent cavities advancing together, one launch per step (lbm_batch_* of include/lbm_amd.h).
+
+    cases: sequence of (nx, ny, nz, Re, u_inf).  tile = (tile_x, tile_y, tile_z), 0 = autotune;
+    persistent = 0/1, -1 = autotune.
+    """
+
+    def __init__(self, cases, velocity_set=19, device=0, tile=(0, 0, 0), persistent=-1):
+        L = _batch_lib()
+        arr = (lbm_batch_case * len(cases))()
+        for i, (nx, ny, nz, Re, u) in enumerate(cases):
+            arr[i].nx, arr[i].ny, arr[i].nz = nx, ny, nz
+            arr[i].Re, arr[i].u_inf = float(Re), float(u)
+        cfg = lbm_batch_config()
+        cfg.velocity_set = velocity_set
+        cfg.device = device
+        cfg.stream = None
+        cfg.tile_x, cfg.tile_y, cfg.tile_z = tile
+        cfg.persistent = persistent
+        b = ctypes.c_void_p()
+        rc = L.lbm_batch_create(ctypes.byref(cfg), arr, len(cases), ctypes.byref(b))
+        if rc != 0:
+            raise LBMError(f"lbm_batch_create failed ({rc}): {L.lbm_batch_create_error().decode()}")
+        self._b = b
+        self.cases = [tuple(c) for c in cases]
+        self.ncases = len(cases)
+        self.Q = velocity_set
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LBMError(f"({rc}) {_batch_lib().lbm_batch_last_error(self._b).decode()}")
+
+    def close(self):
+        if getattr(self, "_b", None):
+            _batch_lib().lbm_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def tiling(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(_batch_lib().lbm_batch_get_tiling(self._b, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    @property
+    def tuning_report(self):
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(_batch_lib().lbm_batch_tuning_report(self._b, buf, len(buf)))
+        return buf.value.decode()
+
+    @property
+    def work_items(self):
+        n = ctypes.c_int()
+        self._check(_batch_lib().lbm_batch_work_items(self._b, ctypes.byref(n)))
+        return n.value
+
+    def set_active(self, active):
+        a = np.ascontiguousarray(np.asarray(active, dtype=np.uint8).reshape(-1))
+        assert a.size == self.ncases
+        self._check(_batch_lib().lbm_batch_set_active(self._b, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+
+    def step(self, n=1):
+        self._check(_batch_lib().lbm_batch_step(self._b, int(n)))
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        self._check(_batch_lib().lbm_batch_step_timed(self._b, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        self._check(_batch_lib().lbm_batch_synchronize(self._b))
+
+    @property
+    def steps(self):
+        out = np.empty(self.ncases, dtype=np.int64)
+        self._check(_batch_lib().lbm_batch_get_steps(self._b, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def _n(self, cs):
+        # an index out of range is left to the library, which reports it
+        if not 0 <= cs < self.ncases:
+            return 1
+        nx, ny, nz = self.cases[cs][:3]
+        return nx * ny * nz
+
+    def set_moments(self, cs, mom, layout=LBM_LAYOUT_GLOBAL):
+        arrs = [np.ascontiguousarray(m, dtype=np.float32).reshape(-1) for m in mom]
+        assert len(arrs) == 10 and all(a.size == self._n(cs) for a in arrs)
+        self._check(_batch_lib().lbm_batch_set_moments(self._b, int(cs), _ptr_array(arrs, ctypes.c_float), layout))
+
+    def get_moments(self, cs, layout=LBM_LAYOUT_GLOBAL):
+        arrs = [np.empty(self._n(cs), dtype=np.float32) for _ in range(10)]
+        self._check(_batch_lib().lbm_batch_get_moments(self._b, int(cs), _ptr_array(arrs, ctypes.c_float), layout))
+        return arrs
+
+    def reduce(self):
+        ke = np.empty(self.ncases)
+        res = np.empty(self.ncases)
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(_batch_lib().lbm_batch_reduce(self._b, ke.ctypes.data_as(dp), res.ctypes.data_as(dp)))
+        return ke, res
+
+    def run(self, tol, check, max_steps):
+        """Steps every active case until its residual < tol or max_steps: (steps, residual, wall ms)."""
+        steps = np.empty(self.ncases, dtype=np.int64)
+        res = np.empty(self.ncases)
+        ms = ctypes.c_double()
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(_batch_lib().lbm_batch_run(self._b, float(tol), int(check), int(max_steps),
+                                               steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                               res.ctypes.data_as(dp), ctypes.byref(ms)))
+        return steps, res, ms.value
+
+
+# ------------------------------------------------------------------ surrogate
+EXPORTED += ["lbm_sur_create", "lbm_sur_destroy", "lbm_sur_last_error", "lbm_sur_create_error",
+             "lbm_sur_num_params", "lbm_sur_get_params", "lbm_sur_set_params", "lbm_sur_encode",
+             "lbm_sur_gradient", "lbm_sur_load_data", "lbm_sur_train", "lbm_sur_predict", "lbm_sur_warm_start"]
+
+
+def _sur_lib():
+    L = lib()
+    if not getattr(L, "_sur_ready", False):
+        P = ctypes.c_void_p
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.lbm_sur_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.lbm_sur_destroy.argtypes = [P]
+        L.lbm_sur_last_error.argtypes = [P]
+        L.lbm_sur_last_error.restype = ctypes.c_char_p
+        L.lbm_sur_create_error.restype = ctypes.c_char_p
+        L.lbm_sur_num_params.argtypes = [P]
+        L.lbm_sur_num_params.restype = ctypes.c_int64
+        L.lbm_sur_get_params.argtypes = [P, fp]
+        L.lbm_sur_set_params.argtypes = [P, fp]
+        L.lbm_sur_encode.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, fp]
+        L.lbm_sur_gradient.argtypes = [P, ctypes.c_int, fp, fp, dp, fp]
+        L.lbm_sur_load_data.argtypes = [P, ctypes.c_int, fp, fp]
+        L.lbm_sur_train.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, dp]
+        L.lbm_sur_predict.argtypes = [P, ctypes.c_int, fp, fp]
+        L.lbm_sur_warm_start.argtypes = [P, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L._sur_ready = True
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def encode_inputs(S, Re, N):
+    """Input channels of one case on an S^3 grid: [S^3][3] (lid mask, c_Re, c_N)."""
+    out = np.empty((S ** 3, 3), dtype=np.float32)
+    rc = _sur_lib().lbm_sur_encode(int(S), float(Re), int(N), _fptr(out))
+    if rc != 0:
+        raise LBMError(f"lbm_sur_encode failed ({rc})")
+    return out
+
+
+class Surrogate:
+    """The 3D U-Net warm-start network of include/lbm_surrogate.h (GPU only)."""
+
+    def __init__(self, S=32, seed=0, device=0):
+        L = _sur_lib()
+        p = ctypes.c_void_p()
+        rc = L.lbm_sur_create(int(device), int(S), int(seed), ctypes.byref(p))
+        if rc != 0:
+            raise LBMError(f"lbm_sur_create failed ({rc}): {L.lbm_sur_create_error().decode()}")
+        self._p = p
+        self.S = S
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LBMError(f"({rc}) {_sur_lib().lbm_sur_last_error(self._p).decode()}")
+
+    def close(self):
+        if getattr(self, "_p", None):
+            _sur_lib().lbm_sur_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def num_params(self):
+        return int(_sur_lib().lbm_sur_num_params(self._p))
+
+    def get_params(self):
+        out = np.empty(self.num_params, dtype=np.float32)
+        self._check(_sur_lib().lbm_sur_get_params(self._p, _fptr(out)))
+        return out
+
+    def set_params(self, params):
+        a = _f32(params).reshape(-1)
+        assert a.size == self.num_params
+        self._check(_sur_lib().lbm_sur_set_params(self._p, _fptr(a)))
+
+    def gradient(self, inputs, targets):
+        """(loss, grad) of the MSE for inputs [n][S^3][3], targets [n][S^3][4]."""
+        x, t = _f32(inputs), _f32(targets)
+        n = x.shape[0]
+        g = np.empty(self.num_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(_sur_lib().lbm_sur_gradient(self._p, n, _fptr(x), _fptr(t), ctypes.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def load_data(self, inputs, targets):
+        x, t = _f32(inputs), _f32(targets)
+        self._check(_sur_lib().lbm_sur_load_data(self._p, x.shape[0], _fptr(x), _fptr(t)))
+
+    def train(self, epochs, batch=8, lr=1e-3, seed=0):
+        losses = np.empty(max(1, epochs))
+        self._check(_sur_lib().lbm_sur_train(self._p, int(epochs), int(batch), float(lr), int(seed),
+                                             losses.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return losses[:epochs]
+
+    def predict(self, inputs):
+        x = _f32(inputs)
+        n = x.shape[0]
+        out = np.empty((n, self.S ** 3, 4), dtype=np.float32)
+        self._check(_sur_lib().lbm_sur_predict(self._p, n, _fptr(x), _fptr(out)))
+        return out
+
+    def warm_start(self, solver, Re, u_inf):
+        self._check(_sur_lib().lbm_sur_warm_start(self._p, solver._h, float(Re), float(u_inf)))

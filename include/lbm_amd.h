@@ -108,10 +108,14 @@ int lbm_step_timed(lbm_handle *h, int n, double *ms);
 int lbm_get_time(const lbm_handle *h, int64_t *t);
 int lbm_set_time(lbm_handle *h, int64_t t);
 
-/* Replaces objectRegistry (objectRegistry.cuh:68-117): enables fused function
- * objects computed inside the step from the post-collision moments. Means use
- * FP64 accumulators and invNewCount = 1/(t+1) of the absolute step t
- * (kineticEnergy.cuh:249).  Enabling resets the means to zero. */
+/* Replaces objectRegistry (objectRegistry.cuh:68-117), whose calculate() runs
+ * after every step (momentBasedD3Q19.cu:133).  The time means (kMean, SMean)
+ * are fused into the step: FP64 accumulators updated from the stored
+ * post-collision moments with invNewCount = 1/(t+1) of the absolute step t
+ * (kineticEnergy.cuh:249).  The instantaneous fields (k, S) are functions of
+ * the current moments alone, so they are formed from them when read, with the
+ * same formulas (kineticEnergy.cuh:71-85, strainRateTensor.cuh:71-84).
+ * Enabling resets the means to zero. */
 int lbm_fo_enable(lbm_handle *h, unsigned mask);
 /* Reads an FO field.  LBM_FIELD_K/KMEAN: n values; S/SMEAN: 6*n values in the
  * order xx,xy,xz,yy,yz,zz, component-major. */
@@ -129,6 +133,74 @@ int lbm_reduce(lbm_handle *h, double *kinetic_energy, double *residual);
  * that population is unknown (source outside the domain).  type_out[d] is the
  * node-type bitmask of src/boundaryConditions/normalVector.cuh:321-334. */
 int lbm_probe_streaming(lbm_handle *h, uint32_t *src_out, uint8_t *type_out);
+
+/* Tile autotuner (no reference counterpart: the reference launches fixed 8x8x8
+ * blocks, momentBasedD3Q19.cu:129 with src/latticeMesh/latticeMesh.cuh:295).
+ * Creates a solver for cfg's case with every supported footprint and plane
+ * count in turn, times `steps` steps of each and returns the fastest.  cfg's
+ * tile fields are ignored. */
+int lbm_autotune(const lbm_config *cfg, int steps, int *tile_x, int *tile_y, int *tile_z, double *ms_per_step);
+
+/* ------------------------------------------------------------------ batches
+ * Many independent cavities of uneven size, Re and lid speed advancing
+ * together: one kernel launch per step covers a tile-to-case table of
+ * (case, column, chunk) work items, and steps are replayed from hipGraphs.
+ * No reference counterpart: the reference runs one case per process
+ * (momentBasedD3Q19.cu:54-137); each case follows the same physics as
+ * lbm_create/lbm_step.  Storage is FP32. */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_batch_case {
+    int nx, ny, nz;
+    double Re, u_inf; /* as in lbm_config */
+} lbm_batch_case;
+
+typedef struct lbm_batch_config {
+    int velocity_set;   /* 19 or 27 */
+    int device;         /* HIP device ordinal */
+    void *stream;       /* hipStream_t, or NULL for a batch-owned stream */
+    int tile_x, tile_y; /* footprint shared by all cases; 0 = autotune */
+    int tile_z;         /* planes per work item: the largest divisor of each case's
+                           nz not above tile_z; 0 = autotune */
+    int persistent;     /* 0: one workgroup per work item; 1: a persistent grid
+                           striding over the items; -1 = autotune */
+} lbm_batch_config;
+
+/* Allocates every case at rest.  With any autotune field set, each candidate
+ * (footprint x plane count x grid kind) is built and timed on the real batch
+ * first; lbm_batch_tuning_report lists them. */
+int lbm_batch_create(const lbm_batch_config *cfg, const lbm_batch_case *cases, int ncases, lbm_batch **out);
+/* Message of the last failure of lbm_batch_create or lbm_autotune (thread-local). */
+const char *lbm_batch_create_error(void);
+int lbm_batch_destroy(lbm_batch *b);
+const char *lbm_batch_last_error(const lbm_batch *b);
+int lbm_batch_get_tiling(const lbm_batch *b, int *tile_x, int *tile_y, int *tile_z, int *persistent);
+/* Text table of the autotuner's candidates and their ms per step (empty when
+ * nothing was tuned).  Writes at most cap bytes including the NUL. */
+int lbm_batch_tuning_report(const lbm_batch *b, char *buf, int cap);
+/* Work items of the active cases (the grid of one step when not persistent). */
+int lbm_batch_work_items(const lbm_batch *b, int *items);
+/* active[i] != 0 keeps case i stepping; an inactive case keeps its fields and
+ * step count.  All cases start active. */
+int lbm_batch_set_active(lbm_batch *b, const uint8_t *active);
+/* Advances every active case n steps (asynchronous). */
+int lbm_batch_step(lbm_batch *b, int n);
+int lbm_batch_step_timed(lbm_batch *b, int n, double *ms);
+int lbm_batch_synchronize(lbm_batch *b);
+/* Steps taken by each case (ncases values). */
+int lbm_batch_get_steps(const lbm_batch *b, int64_t *steps);
+/* Moments of case cs, as lbm_set_moments / lbm_get_moments. */
+int lbm_batch_set_moments(lbm_batch *b, int cs, const float *const m[10], int layout);
+int lbm_batch_get_moments(lbm_batch *b, int cs, float *m[10], int layout);
+/* lbm_reduce for every case at once (ncases values each; either may be NULL). */
+int lbm_batch_reduce(lbm_batch *b, double *kinetic_energy, double *residual);
+/* Runs the sweep: repeatedly advances the active cases `check` steps (fewer
+ * when a case would pass max_steps), reduces, and deactivates each case whose
+ * residual is below tol or that reached max_steps, until none is active.
+ * steps/residual (ncases each, may be NULL) receive each case's final values;
+ * ms the wall time of the whole run. */
+int lbm_batch_run(lbm_batch *b, double tol, int check, int64_t max_steps, int64_t *steps, double *residual,
+                  double *ms);
 
 #ifdef __cplusplus
 }
