@@ -680,4 +680,62 @@ int lbm_sur_warm_start(lbm_sur *s, lbm_handle *h, double Re, double u_inf) {
     });
 }
 
+int lbm_solve_to_residual(lbm_handle *h, lbm_sur *s, double Re, double u_inf, double tol, int check,
+                          int64_t max_steps, lbm_solve_info *info) {
+    if (!h || !info || check <= 0 || max_steps < 0 || !(tol >= 0)) return LBM_ERR_ARG;
+    std::memset(info, 0, sizeof *info);
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    int rc = LBM_OK;
+    double ke = 0, r = INFINITY;
+    int64_t steps = 0;
+    auto ms_since = [&](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+    if (s) {
+        rc = lbm_sur_warm_start(s, h, Re, u_inf);
+        if (rc != LBM_OK) return rc;
+        rc = lbm_synchronize(h);
+        if (rc != LBM_OK) return rc;
+        info->ms_warm = ms_since(t0);
+        info->warm = 1;
+    }
+    // reference field of the first check: the initial state
+    rc = lbm_reduce(h, &ke, &r);
+    if (rc != LBM_OK) return rc;
+    bool first = true;
+    while (steps < max_steps) {
+        const int n = (int)std::min<int64_t>(check, max_steps - steps);
+        rc = lbm_step(h, n);
+        if (rc == LBM_OK) rc = lbm_reduce(h, &ke, &r);
+        if (rc != LBM_OK) return rc;
+        steps += n;
+        if (first) {
+            info->first_residual = r;
+            first = false;
+            if (s && (!std::isfinite(r) || !std::isfinite(ke) || r >= 1.0)) {
+                // reject the warm start: back to rest
+                int nx, ny, nz;
+                lbm::handle_dims(h, &nx, &ny, &nz);
+                const size_t nn = (size_t)nx * ny * nz;
+                std::vector<float> zero(nn, 0.f);
+                const float *mp[10];
+                for (int k = 0; k < 10; k++) mp[k] = zero.data();
+                rc = lbm_set_moments(h, mp, LBM_LAYOUT_GLOBAL);
+                if (rc == LBM_OK) rc = lbm_reduce(h, &ke, &r);
+                if (rc != LBM_OK) return rc;
+                info->warm = 0;
+                info->fell_back = 1;
+                r = INFINITY;
+                continue;
+            }
+        }
+        if (r < tol) break;
+    }
+    rc = lbm_synchronize(h);
+    if (rc != LBM_OK) return rc;
+    info->steps = steps;
+    info->residual = r;
+    info->ms = ms_since(t0);
+    return LBM_OK;
+}
+
 }  // extern "C"

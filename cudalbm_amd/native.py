@@ -632,3 +632,30 @@ class Surrogate:
 
     def warm_start(self, solver, Re, u_inf):
         self._check(_sur_lib().lbm_sur_warm_start(self._p, solver._h, float(Re), float(u_inf)))
+
+
+class lbm_solve_info(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("residual", ctypes.c_double), ("ms", ctypes.c_double),
+                ("ms_warm", ctypes.c_double), ("first_residual", ctypes.c_double), ("warm", ctypes.c_int),
+                ("fell_back", ctypes.c_int)]
+
+
+EXPORTED += ["lbm_solve_to_residual"]
+
+
+def solve_to_residual(solver, Re, u_inf, tol=1e-6, check=100, max_steps=200000, surrogate=None):
+    """Steps `solver` until its residual < tol (optionally warm-started by `surrogate`); returns a dict."""
+    L = _sur_lib()
+    if not getattr(L, "_solve_ready", False):
+        L.lbm_solve_to_residual.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int, ctypes.c_int64,
+                                            ctypes.POINTER(lbm_solve_info)]
+        L._solve_ready = True
+    info = lbm_solve_info()
+    rc = L.lbm_solve_to_residual(solver._h, surrogate._p if surrogate is not None else None, float(Re),
+                                 float(u_inf), float(tol), int(check), int(max_steps), ctypes.byref(info))
+    if rc != 0:
+        where = surrogate if surrogate is not None else solver
+        msg = (L.lbm_sur_last_error(where._p) if surrogate is not None else L.lbm_last_error(solver._h)).decode()
+        raise LBMError(f"lbm_solve_to_residual failed ({rc}): {msg}")
+    return {name: getattr(info, name) for name, _ in lbm_solve_info._fields_}

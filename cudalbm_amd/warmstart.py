@@ -1,0 +1,203 @@
+"""Learned warm start: data generation, training and evaluation.
+
+    python -m cudalbm_amd.warmstart train --out cudalbm_amd/models/unet32.npz
+    python -m cudalbm_amd.warmstart eval  --model cudalbm_amd/models/unet32.npz
+
+Data: lid-driven cavities (D3Q19 FP32) drawn like the batched sweep (Re
+log-uniform, lid speed uniform) on edges N in {32, 48, 64}, solved together by
+the batched solver (lbm_batch_run) until the residual
+||u_t - u_{t-100}|| / ||u_t|| < 1e-6 or a step cap, then resampled onto the
+network's 32^3 grid (network node i at solver coordinate i (N-1)/31) and
+normalised: u/U, v/U, w/U, p/U^2 with p = (rho - 1)/3.
+
+Training: the U-Net of include/lbm_surrogate.h on the GPU (BF16 MFMA forward
+and backward, Adam).  Evaluation: held-out cavities at N = 128 (and 192),
+solved to residual 1e-6 from rest (cold) and from the network's prediction
+(warm, lbm_solve_to_residual with its fallback rule); reports steps, wall time
+and the rel-L2 difference of the two final velocity fields.
+
+All compute runs in the HIP library; this module only orchestrates the C ABI.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import native as N
+
+TRAIN_SEED = 20261203
+EVAL_SEED = 20261204
+S_NET = 32
+
+
+def draw_cases(n, seed, edges, re_range=(100.0, 3200.0), u_range=(0.02, 0.08), edge_p=None):
+    rng = np.random.default_rng(seed)
+    E = rng.choice(np.asarray(edges), size=n, p=edge_p)
+    Re = np.exp(rng.uniform(np.log(re_range[0]), np.log(re_range[1]), size=n))
+    U = rng.uniform(u_range[0], u_range[1], size=n)
+    return [(int(e), float(r), float(u)) for e, r, u in zip(E, Re, U)]
+
+
+def resample(field, N, S=S_NET):
+    """Trilinear samples of an N^3 field (x fastest) at network nodes i (N-1)/(S-1): -> S^3."""
+    f = np.asarray(field, dtype=np.float64).reshape(N, N, N)
+    if N == S:
+        return f.reshape(-1)
+    sc = np.arange(S) * (N - 1) / (S - 1)
+    i0 = np.minimum(N - 2, np.floor(sc).astype(int))
+    fr = sc - i0
+    out = np.zeros((S, S, S))
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                w = ((fr if dz else 1 - fr)[:, None, None] * (fr if dy else 1 - fr)[None, :, None]
+                     * (fr if dx else 1 - fr)[None, None, :])
+                out += w * f[np.ix_(i0 + dz, i0 + dy, i0 + dx)]
+    return out.reshape(-1)
+
+
+def normalised_fields(mom, N, U, S=S_NET):
+    """Stored moments of an N^3 solution -> [S^3][4] (u/U, v/U, w/U, p/U^2)."""
+    u, v, w = (np.asarray(mom[k], dtype=np.float64) / 3.0 for k in (1, 2, 3))
+    p = np.asarray(mom[0], dtype=np.float64) / 3.0
+    cols = [resample(u / U, N, S), resample(v / U, N, S), resample(w / U, N, S), resample(p / (U * U), N, S)]
+    return np.stack(cols, -1).astype(np.float32)
+
+
+def generate(cases, tol=1e-6, check=100, max_steps=60000, log=print):
+    """Solve cases [(N, Re, U)] with one batch per edge; returns (inputs, targets, info)."""
+    inputs, targets, info = [], [], []
+    by_edge = {}
+    for i, c in enumerate(cases):
+        by_edge.setdefault(c[0], []).append(i)
+    results = {}
+    for edge, idx in sorted(by_edge.items()):
+        t0 = time.perf_counter()
+        with N.Batch([(edge, edge, edge, cases[i][1], cases[i][2]) for i in idx], tile=(16, 16, 16),
+                     persistent=0) as b:
+            steps, res, ms = b.run(tol, check, max_steps)
+            for j, i in enumerate(idx):
+                results[i] = (b.get_moments(j), int(steps[j]), float(res[j]))
+        log(f"# edge {edge}: {len(idx)} cases, steps {int(np.min(steps))}..{int(np.max(steps))}, "
+            f"converged {int(np.sum(res < tol))}/{len(idx)}, {time.perf_counter() - t0:.1f} s")
+    dropped = 0
+    for i, (n, Re, U) in enumerate(cases):
+        mom, steps, res = results[i]
+        if not (np.isfinite(res) and all(np.all(np.isfinite(m)) for m in mom)):
+            dropped += 1  # a case the scheme cannot hold at this resolution
+            continue
+        inputs.append(N.encode_inputs(S_NET, Re, n))
+        targets.append(normalised_fields(mom, n, U))
+        info.append({"N": n, "Re": Re, "U": U, "steps": steps, "residual": res})
+    if dropped:
+        log(f"# dropped {dropped} non-finite cases")
+    return np.stack(inputs), np.stack(targets), info
+
+
+def train(inputs, targets, epochs=150, batch=8, lr=1e-3, seed=TRAIN_SEED, log=print):
+    net = N.Surrogate(S=S_NET, seed=seed)
+    net.load_data(inputs, targets)
+    t0 = time.perf_counter()
+    losses = []
+    # step decay of the learning rate in three stages
+    for stage, frac in enumerate((0.6, 0.25, 0.15)):
+        e = max(1, int(round(epochs * frac)))
+        done = 0
+        while done < e:
+            k = min(10, e - done)
+            losses += list(net.train(k, batch=batch, lr=lr * (0.2 ** stage), seed=seed + 100 * stage + done))
+            done += k
+            log(f"# stage {stage} (lr {lr * 0.2 ** stage:.1e}): epoch {len(losses)}, loss {losses[-1]:.3e}, "
+                f"{time.perf_counter() - t0:.0f} s")
+    return net, np.asarray(losses), time.perf_counter() - t0
+
+
+def velocity_rel_l2(a, b):
+    va = np.stack([np.asarray(a[k], np.float64) for k in (1, 2, 3)])
+    vb = np.stack([np.asarray(b[k], np.float64) for k in (1, 2, 3)])
+    return float(np.linalg.norm(va - vb) / max(np.linalg.norm(vb), 1e-300))
+
+
+def evaluate(net, cases, tol=1e-6, check=100, max_steps=200000, log=print):
+    """Cold and warm solves of each (N, Re, U); returns a list of per-case dicts."""
+    out = []
+    for n, Re, U in cases:
+        with N.Solver(n, n, n, Re, U) as s:
+            cold = N.solve_to_residual(s, Re, U, tol, check, max_steps)
+            mc = s.get_moments()
+        with N.Solver(n, n, n, Re, U) as s:
+            warm = N.solve_to_residual(s, Re, U, tol, check, max_steps, surrogate=net)
+            mw = s.get_moments()
+        r = {"N": n, "Re": Re, "U": U, "cold": cold, "warm": warm, "final_rel_l2": velocity_rel_l2(mw, mc),
+             "time_ratio": warm["ms"] / cold["ms"], "step_ratio": warm["steps"] / max(cold["steps"], 1)}
+        log(f"# N {n} Re {Re:.0f} U {U:.3f}: cold {cold['steps']} steps {cold['ms']:.0f} ms (r {cold['residual']:.1e}), "
+            f"warm {warm['steps']} steps {warm['ms']:.0f} ms (r {warm['residual']:.1e}, first {warm['first_residual']:.2e}, "
+            f"kept {warm['warm']}), time ratio {r['time_ratio']:.3f}, final rel-L2 {r['final_rel_l2']:.2e}")
+        out.append(r)
+    return out
+
+
+def save_model(path, net, meta):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez_compressed(path, params=net.get_params(), meta=json.dumps(meta))
+
+
+def load_model(path):
+    z = np.load(path, allow_pickle=False)
+    net = N.Surrogate(S=S_NET, seed=0)
+    net.set_params(z["params"])
+    return net, json.loads(str(z["meta"]))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", required=True)
+    t.add_argument("--cases", type=int, default=512)
+    t.add_argument("--max-steps", type=int, default=60000)
+    t.add_argument("--epochs", type=int, default=150)
+    t.add_argument("--batch", type=int, default=8)
+    t.add_argument("--lr", type=float, default=1e-3)
+    e = sub.add_parser("eval")
+    e.add_argument("--model", required=True)
+    e.add_argument("--edges", default="128")
+    e.add_argument("--cases", type=int, default=4)
+    e.add_argument("--max-steps", type=int, default=200000)
+    e.add_argument("--json", default="")
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    if a.cmd == "train":
+        cases = draw_cases(a.cases, TRAIN_SEED, (32, 48, 64), edge_p=(0.5, 0.3, 0.2))
+        t0 = time.perf_counter()
+        x, y, info = generate(cases, max_steps=a.max_steps, log=log)
+        gen_s = time.perf_counter() - t0
+        net, losses, train_s = train(x, y, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+        meta = {"cases": len(cases), "seed": TRAIN_SEED, "generate_s": gen_s, "train_s": train_s,
+                "final_loss": float(losses[-1]), "converged": int(sum(i["residual"] < 1e-6 for i in info)),
+                "epochs": len(losses), "batch": a.batch, "lr": a.lr}
+        save_model(a.out, net, meta)
+        print(json.dumps(meta), flush=True)
+    else:
+        net, meta = load_model(a.model)
+        edges = tuple(int(v) for v in a.edges.split(","))
+        cases = draw_cases(a.cases, EVAL_SEED, edges, re_range=(400.0, 2500.0))
+        res = evaluate(net, cases, max_steps=a.max_steps, log=log)
+        summary = {"cases": len(res),
+                   "median_time_ratio": float(np.median([r["time_ratio"] for r in res])),
+                   "median_step_ratio": float(np.median([r["step_ratio"] for r in res])),
+                   "max_final_rel_l2": float(max(r["final_rel_l2"] for r in res)),
+                   "fell_back": int(sum(r["warm"]["fell_back"] for r in res)), "per_case": res}
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(summary, f)
+        print(json.dumps({k: v for k, v in summary.items() if k != "per_case"}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

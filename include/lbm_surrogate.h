@@ -65,6 +65,26 @@ int lbm_sur_predict(lbm_sur *s, int n, const float *inputs, float *fields);
  * solver must be cubic. */
 int lbm_sur_warm_start(lbm_sur *s, lbm_handle *h, double Re, double u_inf);
 
+typedef struct lbm_solve_info {
+    int64_t steps;        /* steps taken, including any discarded warm-start steps */
+    double residual;      /* last residual */
+    double ms;            /* wall time: prediction, conversion, upload and stepping */
+    double ms_warm;       /* part of ms spent on prediction, conversion and upload */
+    double first_residual; /* residual at the first check */
+    int warm;             /* 1 if the run kept the warm start */
+    int fell_back;        /* 1 if the warm start was rejected (NaN, or first residual >= 1) */
+} lbm_solve_info;
+
+/* Runs a solver (at its current state when s is NULL, e.g. rest) until the
+ * velocity residual r = ||u_t - u_{t-check}|| / ||u_t|| (FP64, lbm_reduce)
+ * drops below tol or max_steps steps were taken.  With s, the solver is first
+ * warm-started from the network; the warm start is judged at the first check:
+ * a NaN or a residual >= 1 (what a start from rest shows at its first check,
+ * since u_0 = 0) rejects it, the moments are reset to rest and the run
+ * continues from there (the discarded steps still count in steps and ms). */
+int lbm_solve_to_residual(lbm_handle *h, lbm_sur *s, double Re, double u_inf, double tol, int check,
+                          int64_t max_steps, lbm_solve_info *info);
+
 #ifdef __cplusplus
 }
 #endif
