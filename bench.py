@@ -33,6 +33,40 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X published HBM3E bandwidth
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+MODEL_FILE = os.path.join(ROOT, "cudalbm_amd", "models", "unet32.npz")
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel/workload from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs of rocprofv3, calibrated on a device copy)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)[key]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def warm_vs_cold(n_cases=2, edge=128, max_steps=150000):
+    """Held-out cavities at edge^3 (Re log-uniform [400, 2500]) solved to residual
+    1e-6 from rest and from the trained surrogate's prediction."""
+    import numpy as np
+
+    from cudalbm_amd import warmstart as W
+
+    net, meta = W.load_model(MODEL_FILE)
+    with net:
+        cases = W.draw_cases(n_cases, W.EVAL_SEED, (edge,), re_range=(400.0, 2500.0))
+        res = W.evaluate(net, cases, max_steps=max_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
+    return {"cases": [{"N": r["N"], "Re": round(r["Re"], 1), "U": round(r["U"], 4),
+                       "cold_steps": r["cold"]["steps"], "cold_ms": round(r["cold"]["ms"], 1),
+                       "cold_residual": r["cold"]["residual"], "warm_steps": r["warm"]["steps"],
+                       "warm_ms": round(r["warm"]["ms"], 1), "warm_residual": r["warm"]["residual"],
+                       "warm_kept": bool(r["warm"]["warm"]), "time_ratio": round(r["time_ratio"], 4),
+                       "final_velocity_rel_l2": r["final_rel_l2"]} for r in res],
+            "median_time_ratio": float(np.median([r["time_ratio"] for r in res])),
+            "max_final_rel_l2": float(max(r["final_rel_l2"] for r in res)),
+            "model": {k: meta[k] for k in ("cases", "epochs", "final_loss") if k in meta}}
 
 
 def measure(n, Q, storage, steps, warmup, fo_mask=0, Re=1000.0, u=0.05, tile=(0, 0, 0)):
@@ -121,8 +155,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="only the headline D3Q19 FP32 step")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-vs-cold leg")
     ap.add_argument("--only", default="",
-                    help="profile one variant: q19fo|q19fp32|q19fp16|q27fp32|q27fp16|batch")
+                    help="profile one variant: q19fo|q19fp32|q19fp16|q27fp32|q27fp16|batch|warm")
     ap.add_argument("--tile", default="0,0,0")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (x4), to calibrate HBM byte counters")
@@ -148,6 +183,9 @@ def main():
         if args.only == "batch":
             print(json.dumps({"variant": "batch", **batch_sweep(args.steps)}), flush=True)
             return
+        if args.only == "warm":
+            print(json.dumps({"variant": "warm", **warm_vs_cold()}), flush=True)
+            return
         spec = {"q19fo": (19, "fp32", FO_ALL), "q19fp32": (19, "fp32", 0), "q19fp16": (19, "fp16", 0),
                 "q27fp32": (27, "fp32", 0), "q27fp16": (27, "fp16", 0)}[args.only]
         r = measure(args.n, spec[0], spec[1], args.steps, args.warmup, fo_mask=spec[2], tile=tile)
@@ -169,6 +207,8 @@ def main():
             print(f"# {name}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         variants["batch_sweep64_d3q19_fp32"] = batch_sweep(100)
         print(f"# batch sweep: {variants['batch_sweep64_d3q19_fp32']}", file=sys.stderr, flush=True)
+        if os.path.exists(MODEL_FILE) and not args.no_warm:
+            variants["warm_vs_cold_time_to_residual_1e-6"] = warm_vs_cold()
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "MLUPS (D3Q19 256^3 lid-driven cavity, FP32, k/kMean/S/SMean on)",
@@ -187,7 +227,8 @@ def main():
                                "function objects k, kMean, S, SMean on",
                    "velocity_set": "D3Q19", "nodes": args.n ** 3, "tiling": head["tiling"]},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / PEAK_HBM_GBS, 4),
+                     "traffic": measured_traffic(f"step_kernel<19,32,16,float,fo>@{args.n}^3"),
                      "bytes_per_node": head["bytes_per_node"]},
         "cpu_baseline": cpu,
         "variants": variants,

@@ -12,7 +12,9 @@
  * Layouts: grids are S^3 with S a multiple of 8 (S = 32 for the trained
  * network), x fastest then y then z, channels last:
  *   inputs  [n][S^3][3]  (lid mask, Re code, N code; see lbm_sur_encode)
- *   fields  [n][S^3][4]  (u/U, v/U, w/U, p/U^2 with p = (rho-1)/3)
+ *   fields  [n][S^3][4]  (u/U, v/U, w/U, p/U^2 with p = (rho-1)/3 minus its
+ *                         domain mean: the cavity's mean density drifts with
+ *                         the elapsed steps and is not predicted)
  * Every function returns 0 or a negative lbm_status (include/lbm_amd.h).
  */
 #ifndef LBM_SURROGATE_H
