@@ -4,7 +4,7 @@
     python -m cudalbm_amd.warmstart eval  --model cudalbm_amd/models/unet32.npz
 
 Data: lid-driven cavities (D3Q19 FP32) drawn like the batched sweep (Re
-log-uniform, lid speed uniform) on edges N in {32, 48, 64}, solved together by
+log-uniform, lid speed uniform) on edges N in {32, 48, 64, 96, 128}, solved together by
 the batched solver (lbm_batch_run) until the residual
 ||u_t - u_{t-100}|| / ||u_t|| < 1e-6 or a step cap, then resampled onto the
 network's 32^3 grid (network node i at solver coordinate i (N-1)/31) and
@@ -33,6 +33,9 @@ from . import native as N
 TRAIN_SEED = 20261203
 EVAL_SEED = 20261204
 S_NET = 32
+# training edges: the sweep's range up to 128, weighted towards the cheap ones
+TRAIN_EDGES = (32, 48, 64, 96, 128)
+TRAIN_EDGE_P = (0.25, 0.25, 0.25, 0.15, 0.10)
 
 
 def draw_cases(n, seed, edges, re_range=(100.0, 3200.0), u_range=(0.02, 0.08), edge_p=None):
@@ -179,7 +182,7 @@ def main(argv=None):
     t = sub.add_parser("train")
     t.add_argument("--out", required=True)
     t.add_argument("--cases", type=int, default=512)
-    t.add_argument("--max-steps", type=int, default=60000)
+    t.add_argument("--max-steps", type=int, default=80000)
     t.add_argument("--epochs", type=int, default=150)
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--lr", type=float, default=1e-3)
@@ -192,12 +195,12 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "train":
-        cases = draw_cases(a.cases, TRAIN_SEED, (32, 48, 64), edge_p=(0.5, 0.3, 0.2))
+        cases = draw_cases(a.cases, TRAIN_SEED, TRAIN_EDGES, edge_p=TRAIN_EDGE_P)
         t0 = time.perf_counter()
         x, y, info = generate(cases, max_steps=a.max_steps, log=log)
         gen_s = time.perf_counter() - t0
         net, losses, train_s = train(x, y, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
-        meta = {"cases": len(cases), "seed": TRAIN_SEED, "generate_s": gen_s, "train_s": train_s,
+        meta = {"cases": len(cases), "seed": TRAIN_SEED, "edges": TRAIN_EDGES, "generate_s": gen_s, "train_s": train_s,
                 "final_loss": float(losses[-1]), "converged": int(sum(i["residual"] < 1e-6 for i in info)),
                 "epochs": len(losses), "batch": a.batch, "lr": a.lr}
         save_model(a.out, net, meta)

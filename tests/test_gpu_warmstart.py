@@ -1,0 +1,69 @@
+"""What the trained surrogate gains: a held-out cavity solved to residual 1e-6
+from rest and from the network's prediction (the committed model
+cudalbm_amd/models/unet32.npz), plus the fallback rule on a broken network.
+
+Bounds: the warm start takes <= 0.7x the cold steps (measured ~0.5x at 64^3)
+and both final velocity fields agree to rel-L2 <= 5e-4 (two runs stopped at
+r = 1e-6 approach the steady state from different sides).  A network whose
+prediction is garbage must be rejected at the first check and the run must
+then match the cold run to the same bound.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODEL = os.path.join(ROOT, "cudalbm_amd", "models", "unet32.npz")
+CASE = (64, 600.0, 0.05)  # N, Re, lid speed
+
+
+def vel(m):
+    return np.stack([np.asarray(m[k], np.float64) / 3.0 for k in (1, 2, 3)])
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def solve(net=None):
+    from cudalbm_amd import Solver
+    from cudalbm_amd.native import solve_to_residual
+
+    n, Re, U = CASE
+    with Solver(n, n, n, Re, U) as s:
+        info = solve_to_residual(s, Re, U, 1e-6, 100, 150000, surrogate=net)
+        return info, s.get_moments()
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained model in the tree")
+def test_trained_warm_start_beats_cold(native_lib):
+    from cudalbm_amd import warmstart as W
+
+    cold, mc = solve()
+    net, meta = W.load_model(MODEL)
+    with net:
+        warm, mw = solve(net)
+    print(f"cold {cold['steps']} steps {cold['ms']:.0f} ms; warm {warm['steps']} steps {warm['ms']:.0f} ms "
+          f"(first residual {warm['first_residual']:.3f}); final velocity rel-L2 {rel(vel(mw), vel(mc)):.2e}",
+          flush=True)
+    assert cold["residual"] < 1e-6 and warm["residual"] < 1e-6
+    assert warm["warm"] == 1 and warm["fell_back"] == 0
+    assert warm["steps"] <= 0.7 * cold["steps"]
+    assert rel(vel(mw), vel(mc)) <= 5e-4
+
+
+def test_broken_network_falls_back(native_lib):
+    from cudalbm_amd import Surrogate
+
+    cold, mc = solve()
+    with Surrogate(S=32, seed=1) as net:
+        net.set_params(net.get_params() * 50.0)  # predictions far off the flow
+        warm, mw = solve(net)
+    print(f"fallback: first residual {warm['first_residual']:.3g}, steps {warm['steps']} (cold {cold['steps']})",
+          flush=True)
+    assert warm["fell_back"] == 1 and warm["warm"] == 0
+    assert warm["steps"] == cold["steps"] + 100  # the rejected check costs one check interval
+    assert rel(vel(mw), vel(mc)) <= 5e-4
