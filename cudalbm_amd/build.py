@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblbm_amd.so")
 
 HIP_SOURCES = ["lbm_capi.hip", "lbm_batch.hip", "surrogate.hip"]
-HOST_SOURCES = ["case_io.cpp"]
+HOST_SOURCES = ["case_io.cpp", "postprocess.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -64,17 +64,20 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-APPS = {"momentBasedD3Q19": "momentBasedD3Q19.cpp", "momentBasedD3Q27": "momentBasedD3Q27.cpp"}
+APPS = {"momentBasedD3Q19": ["momentBased.cpp", "momentBasedD3Q19.cpp"],
+        "momentBasedD3Q27": ["momentBased.cpp", "momentBasedD3Q27.cpp"],
+        "fieldConvert": ["fieldTools.cpp", "fieldConvert.cpp"],
+        "fieldCalculate": ["fieldTools.cpp", "fieldCalculate.cpp"]}
 BIN = os.path.join(HERE, "bin")
 
 
 def build_apps(verbose: bool = False) -> None:
-    """Solver drivers, linked against liblbm_amd.so (found via $ORIGIN/..)."""
+    """Solver and post-processing drivers, linked against liblbm_amd.so (found via $ORIGIN/..)."""
     os.makedirs(BIN, exist_ok=True)
     apps = os.path.join(CSRC, "apps")
-    for exe, main in APPS.items():
+    for exe, srcs in APPS.items():
         cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-               os.path.join(apps, "momentBased.cpp"), os.path.join(apps, main),
+               *[os.path.join(apps, f) for f in srcs],
                "-L", HERE, "-llbm_amd", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(BIN, exe)]
         if verbose:
             print(" ".join(cmd), flush=True)

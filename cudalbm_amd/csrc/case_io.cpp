@@ -335,6 +335,11 @@ int guard(F &&f) {
 
 }  // namespace
 
+// Shared with the post-processing unit (same thread-local message).
+namespace lbm {
+void case_set_error(const std::string &msg) { g_err = msg; }
+}  // namespace lbm
+
 extern "C" {
 
 const char *lbm_case_last_error(void) { return g_err.c_str(); }

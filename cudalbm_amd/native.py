@@ -659,3 +659,81 @@ def solve_to_residual(solver, Re, u_inf, tol=1e-6, check=100, max_steps=200000, 
         msg = (L.lbm_sur_last_error(where._p) if surrogate is not None else L.lbm_last_error(solver._h)).decode()
         raise LBMError(f"lbm_solve_to_residual failed ({rc}): {msg}")
     return {name: getattr(info, name) for name, _ in lbm_solve_info._fields_}
+
+
+# ------------------------------------------------------------------ post-processing (host only)
+LBM_POST_VTU, LBM_POST_VTS, LBM_POST_TECPLOT = 0, 1, 2
+LBM_POST_CENTRAL, LBM_POST_REFERENCE = 0, 1
+EXPORTED += ["lbm_post_time_indices", "lbm_post_write", "lbm_post_derivative", "lbm_post_curl", "lbm_post_div",
+             "lbm_post_cut_plane"]
+
+
+def _post_lib():
+    L = _case_lib()
+    if not getattr(L, "_post_ready", False):
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i = ctypes.c_int
+        L.lbm_post_time_indices.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), i]
+        L.lbm_post_write.argtypes = [i, ctypes.c_char_p, i, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_double, i,
+                                     ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(fp)]
+        L.lbm_post_derivative.argtypes = [i, i, i, i, i, i, fp, dp]
+        L.lbm_post_curl.argtypes = [i, i, i, i, i, fp, fp, fp, fp, fp, fp]
+        L.lbm_post_div.argtypes = [i, i, i, i, i, fp, fp, fp, fp]
+        L.lbm_post_cut_plane.argtypes = [i, ctypes.c_double, i, i, i, ctypes.c_double, fp, fp]
+        L._post_ready = True
+    return L
+
+
+def post_time_indices(directory, prefix):
+    L = _post_lib()
+    n = L.lbm_post_time_indices(str(directory).encode(), prefix.encode(), None, 0)
+    _case_check(0 if n >= 0 else n)
+    out = np.empty(n, dtype=np.int64)
+    if n:
+        L.lbm_post_time_indices(str(directory).encode(), prefix.encode(),
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), n)
+    return list(out)
+
+
+def post_write(fmt, path, shape, L_xyz, names, fields):
+    nx, ny, nz = shape
+    arrs = [_f32(f).reshape(-1) for f in fields]
+    cn = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+    _case_check(_post_lib().lbm_post_write(fmt, str(path).encode(), nx, ny, nz, *map(float, L_xyz), len(names), cn,
+                                           _ptr_array(arrs, ctypes.c_float)))
+
+
+def post_derivative(f, shape, axis, order=8, scheme=LBM_POST_CENTRAL):
+    nx, ny, nz = shape
+    a = _f32(f).reshape(-1)
+    out = np.empty(a.size)
+    _case_check(_post_lib().lbm_post_derivative(scheme, order, axis, nx, ny, nz, _fptr(a),
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def post_curl(u, v, w, shape, order=8, scheme=LBM_POST_CENTRAL):
+    nx, ny, nz = shape
+    a, b, c = (_f32(x).reshape(-1) for x in (u, v, w))
+    o = [np.empty(a.size, dtype=np.float32) for _ in range(3)]
+    _case_check(_post_lib().lbm_post_curl(scheme, order, nx, ny, nz, _fptr(a), _fptr(b), _fptr(c), *map(_fptr, o)))
+    return o
+
+
+def post_div(u, v, w, shape, order=8, scheme=LBM_POST_CENTRAL):
+    nx, ny, nz = shape
+    a, b, c = (_f32(x).reshape(-1) for x in (u, v, w))
+    o = np.empty(a.size, dtype=np.float32)
+    _case_check(_post_lib().lbm_post_div(scheme, order, nx, ny, nz, _fptr(a), _fptr(b), _fptr(c), _fptr(o)))
+    return o
+
+
+def post_cut_plane(f, shape, axis, coord, L=1.0):
+    nx, ny, nz = shape
+    a = _f32(f).reshape(-1)
+    m = [nx, ny, nz]
+    m[axis] = 1
+    out = np.empty(m[0] * m[1] * m[2], dtype=np.float32)
+    _case_check(_post_lib().lbm_post_cut_plane(axis, float(coord), nx, ny, nz, float(L), _fptr(a), _fptr(out)))
+    return out
