@@ -18,9 +18,13 @@ static __global__ void decode_f16_kernel(const _Float16 *__restrict__ src, float
         dst[i] = float(src[i]) * dec;
 }
 
-// Advances the absolute step counter read by the fused time averages.
-static __global__ void tick_kernel(int64_t *t) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) t[0] += 1;
+// Advances the step counter read by the fused time averages.
+static __global__ void tick_kernel(StepClock *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t t = c->t + 1;
+        c->t = t;
+        c->inv = 1.0 / double(t + 1);
+    }
 }
 
 static __global__ void fill_kernel_f64(double *p, int64_t n, double v) {
@@ -38,9 +42,9 @@ __global__ void fo_instant_kernel(const StepParams p, float *__restrict__ k, flo
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < p.n; i += (int64_t)gridDim.x * blockDim.x) {
         float m[10];
 #pragma unroll
-        for (int c = 0; c < 10; c++) m[c] = Store<ST>::load(p.mom[c], (int)i, p.dec[c]);
+        for (int c = 0; c < 10; c++) m[c] = Store<ST>::load(p.mom[c], (unsigned)i * Store<ST>::kBytes, p.dec[c]);
         float kk, s[6];
-        fo_values(m, p.tau, kk, s);
+        fo_values(m, p.s_diag, p.s_off, kk, s);
         if (k) k[i] = kk;
         if (S) {
 #pragma unroll
@@ -74,10 +78,10 @@ __device__ __forceinline__ void residual_partials(const StepParams &p, float *__
     double ke = 0.0, du = 0.0, uu = 0.0;
     for (int64_t i = j * (int64_t)NT + threadIdx.x; i < p.n; i += (int64_t)nb * NT) {
         // physical rho and u formed in FP64 from the stored values
-        const double rho = 1.0 + (double)Store<ST>::load(p.mom[0], (int)i, p.dec[0]);
+        const double rho = 1.0 + (double)Store<ST>::load(p.mom[0], (unsigned)i * Store<ST>::kBytes, p.dec[0]);
         double u[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) u[a] = (double)Store<ST>::load(p.mom[1 + a], (int)i, p.dec[1 + a]) / 3.0;
+        for (int a = 0; a < 3; a++) u[a] = (double)Store<ST>::load(p.mom[1 + a], (unsigned)i * Store<ST>::kBytes, p.dec[1 + a]) / 3.0;
         double uu_i = 0.0, du_i = 0.0;
 #pragma unroll
         for (int a = 0; a < 3; a++) {

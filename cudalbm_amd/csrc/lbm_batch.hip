@@ -387,7 +387,8 @@ void build(lbm_batch *b, const lbm_batch_case *cs, int ncases) {
             throw ArgError{LBM_ERR_SIZE, tag + "nx and ny must be multiples of the tile footprint " +
                                              std::to_string(b->TX) + "x" + std::to_string(b->TY)};
         c.n = (int64_t)k.nx * k.ny * k.nz;
-        if (c.n >= (int64_t)INT32_MAX) throw ArgError{LBM_ERR_SIZE, tag + "mesh too large: node indices are 32-bit"};
+        if (c.n > ((int64_t)1 << 30))
+            throw ArgError{LBM_ERR_SIZE, tag + "mesh too large: at most 2^30 nodes (4 GiB per moment array)"};
         c.lz = lz_for(k.nz, b->LZT);
         c.nbx = k.nx / b->TX;
         c.nby = k.ny / b->TY;

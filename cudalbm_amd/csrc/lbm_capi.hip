@@ -175,7 +175,7 @@ struct lbm_handle {
     unsigned fo_mask = 0;
     float *k = nullptr, *S = nullptr;
     double *kmean = nullptr, *smean = nullptr;
-    int64_t *tcount = nullptr;
+    StepClock *clock = nullptr;
     int64_t t = 0;
     float *uprev = nullptr;
     double *part = nullptr, *red_out = nullptr;
@@ -215,7 +215,9 @@ struct lbm_handle {
         p.fo_mask = probe ? 0u : fo_mask;
         p.kmean = kmean;
         p.smean = smean;
-        p.tcount = tcount;
+        p.clock = clock;
+        p.s_diag = (float)(1.0 / (3.0 * tau));
+        p.s_off = (float)(1.0 / (6.0 * tau));
         p.n = n;
         return p;
     }
@@ -248,12 +250,12 @@ struct lbm_handle {
             if (f) (void)hipFree(f);
             f = nullptr;
         }
-        for (void *p : {(void *)k, (void *)S, (void *)kmean, (void *)smean, (void *)tcount, (void *)uprev,
+        for (void *p : {(void *)k, (void *)S, (void *)kmean, (void *)smean, (void *)clock, (void *)uprev,
                         (void *)part, (void *)red_out})
             if (p) (void)hipFree(p);
         k = S = nullptr;
         kmean = smean = nullptr;
-        tcount = nullptr;
+        clock = nullptr;
         uprev = nullptr;
         part = red_out = nullptr;
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -357,14 +359,24 @@ void check_layout(lbm_handle *h, int layout) {
         throw ArgError{LBM_ERR_SIZE, "TILED8 layout needs nx, ny, nz multiples of 8"};
 }
 
+// Device step clock <- the handle's time t (stream-ordered; the caller syncs).
+void set_clock(lbm_handle *h) {
+    static thread_local StepClock c;  // stays valid until the copy has run
+    c.t = h->t;
+    c.inv = 1.0 / double(h->t + 1);
+    HIPX(hipMemcpyAsync(h->clock, &c, sizeof c, hipMemcpyHostToDevice, h->stream));
+    HIPX(hipStreamSynchronize(h->stream));
+}
+
 // One step, launched on the handle's stream (captured or not).
 void enqueue_step(lbm_handle *h, int parity) {
     StepParams p = h->params();
     p.fin = h->faces[parity];
     p.fout = h->faces[parity ^ 1];
     launch_step(h->Q, h->TX, h->TY, p, h->grid(), h->stream, h->cfg.storage, h->means(), false);
-    if (h->means()) {
-        hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, h->stream, h->tcount);
+    // the step counter feeds the time means and the FP16 stochastic rounding
+    if (h->means() || h->cfg.storage == LBM_STORE_FP16) {
+        hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, h->stream, h->clock);
         HIPX(hipGetLastError());
     }
 }
@@ -450,7 +462,8 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     h->nby = c.ny / h->TY;
     h->nc = c.nz / h->LZ;
     h->n = (int64_t)c.nx * c.ny * c.nz;
-    if (h->n >= (int64_t)INT32_MAX) return bad(LBM_ERR_SIZE, "mesh too large: node indices are 32-bit");
+    // moment arrays are addressed with 32-bit byte offsets (step_kernel.hpp, Store)
+    if (h->n > ((int64_t)1 << 30)) return bad(LBM_ERR_SIZE, "mesh too large: at most 2^30 nodes (4 GiB per moment array)");
     relaxation(c.Re, c.u_inf, c.nx, &h->omega, &h->tau);
     try {
         HIPX(hipSetDevice(c.device));
@@ -470,8 +483,8 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
             HIPX(hipMalloc(&h->faces[s], sizeof(float) * h->fwords));
             HIPX(hipMemsetAsync(h->faces[s], 0, sizeof(float) * h->fwords, h->stream));
         }
-        HIPX(hipMalloc(&h->tcount, sizeof(int64_t)));
-        HIPX(hipMemsetAsync(h->tcount, 0, sizeof(int64_t), h->stream));
+        HIPX(hipMalloc(&h->clock, sizeof(StepClock)));
+        set_clock(h);
         rebuild_faces(h);
         HIPX(hipStreamSynchronize(h->stream));
     } catch (const HipError &e) {
@@ -617,7 +630,7 @@ int lbm_set_time(lbm_handle *h, int64_t t) {
     return guarded(h, [&] {
         if (t < 0) throw ArgError{LBM_ERR_ARG, "negative time"};
         h->t = t;
-        HIPX(hipMemcpyAsync(h->tcount, &h->t, sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+        set_clock(h);
         HIPX(hipStreamSynchronize(h->stream));
     });
 }
@@ -640,7 +653,7 @@ int lbm_fo_enable(lbm_handle *h, unsigned mask) {
             hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, h->smean, 6 * n, 0.0);
             HIPX(hipGetLastError());
         }
-        HIPX(hipMemcpyAsync(h->tcount, &h->t, sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+        set_clock(h);
         HIPX(hipStreamSynchronize(h->stream));
         h->fo_mask = mask;
     });

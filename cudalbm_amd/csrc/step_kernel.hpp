@@ -36,6 +36,13 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Step counter read by the fused time means: t of the step about to run and
+// invNewCount = 1/(t+1) (kineticEnergy.cuh:249), advanced by tick_kernel.
+struct StepClock {
+    int64_t t;
+    double inv;
+};
+
 struct StepParams {
     int nx, ny, nz, lz, nbx, nby, nc;
     float omega, t_omega, omega_h, u_lid, tau;
@@ -48,7 +55,8 @@ struct StepParams {
     unsigned fo_mask;
     double *kmean;
     double *smean;         // 6 component arrays, each of n doubles
-    const int64_t *tcount; // absolute step index of this step (device counter)
+    const StepClock *clock; // absolute step index of this step and 1/(t+1) (device)
+    float s_diag, s_off;    // strain-rate coefficients 1/(3 tau), 1/(6 tau)
     int64_t n;             // nodes
     // probe outputs
     uint32_t *probe_src;
@@ -57,22 +65,34 @@ struct StepParams {
 
 template <typename ST>
 struct Store;
+// Element i of a moment array is addressed as (array base) + (32-bit byte
+// offset): the base is a kernel argument in scalar registers, so a load or
+// store maps to the scalar-base + vector-offset form of the global memory
+// instructions, and one offset per node serves all ten arrays (no 64-bit
+// address arithmetic per access).  The host keeps every moment array under
+// 4 GiB (at most 2^30 nodes).
+__device__ __forceinline__ const void *byte_at(const void *p, unsigned off) {
+    return static_cast<const char *>(p) + off;
+}
+__device__ __forceinline__ void *byte_at(void *p, unsigned off) { return static_cast<char *>(p) + off; }
 template <>
 struct Store<float> {
-    __device__ static __forceinline__ float load(const void *p, int i, float) {
-        return __builtin_nontemporal_load(static_cast<const float *>(p) + i);
+    static constexpr unsigned kBytes = 4;
+    __device__ static __forceinline__ float load(const void *p, unsigned off, float) {
+        return __builtin_nontemporal_load(static_cast<const float *>(byte_at(p, off)));
     }
-    __device__ static __forceinline__ void store(void *p, int i, float v, float) {
-        __builtin_nontemporal_store(v, static_cast<float *>(p) + i);
+    __device__ static __forceinline__ void store(void *p, unsigned off, float v) {
+        __builtin_nontemporal_store(v, static_cast<float *>(byte_at(p, off)));
     }
 };
 template <>
 struct Store<_Float16> {
-    __device__ static __forceinline__ float load(const void *p, int i, float dec) {
-        return float(static_cast<const _Float16 *>(p)[i]) * dec;
+    static constexpr unsigned kBytes = 2;
+    __device__ static __forceinline__ float load(const void *p, unsigned off, float dec) {
+        return float(*static_cast<const _Float16 *>(byte_at(p, off))) * dec;
     }
-    __device__ static __forceinline__ void store(void *p, int i, float v, float enc) {
-        static_cast<_Float16 *>(p)[i] = _Float16(v * enc);
+    __device__ static __forceinline__ void store(void *p, unsigned off, _Float16 v) {
+        *static_cast<_Float16 *>(byte_at(p, off)) = v;
     }
 };
 
@@ -335,9 +355,9 @@ __device__ __forceinline__ unsigned plane_base(const StepParams &p, int bx, int 
 template <typename ST, int TX, int TY>
 __device__ __forceinline__ void load_plane(const StepParams &p, int bx, int by, int z, int tid,
                                            float (&m)[10]) {
-    const int i = (int)plane_base<TX, TY>(p, bx, by, z) + tid;
+    const unsigned off = (plane_base<TX, TY>(p, bx, by, z) + (unsigned)tid) * Store<ST>::kBytes;
 #pragma unroll
-    for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], i, p.dec[k]);
+    for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], off, p.dec[k]);
 }
 
 // Next plane's old moments, or past the chunk's last plane the down-going
@@ -362,17 +382,42 @@ __device__ __forceinline__ void load_next(const StepParams &p, const float *__re
     }
 }
 
+// FP16 storage rounds the stored 3u stochastically (v_cvt_sr_f16_f32: up with
+// probability equal to the remainder's fraction of an ulp): round-to-nearest
+// of a slowly changing field errs the same way step after step and the error
+// grows like a bias, while stochastic rounding is unbiased.  The random word of
+// node (x, y, z) at step t is seed(x, y, t) + z * 0x9E3779B9 (a Weyl sequence
+// down the column; seed = lowbias32 hash, C. Wellons, of the column index and
+// of t), rotated by 0/11/22 bits for u, v, w: reproducible, independent of the
+// tiling, and one add per node in the plane loop.
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ _Float16 sr_half(float v, unsigned r) {
+    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+    half2_t o = {(_Float16)0.f, (_Float16)0.f};
+    o = __builtin_amdgcn_cvt_sr_f16_f32(o, v, r, false);
+    return o[0];
+}
+
 // Function-object values of one node from its stored (scaled) moments:
-// k = kineticEnergy.cuh:71-85, S = strainRateTensor.cuh:71-84 (tau = 1/omega).
-__device__ __forceinline__ void fo_values(const float m[10], float tau, float &kk, float S[6]) {
+// k = kineticEnergy.cuh:71-85; S = strainRateTensor.cuh:71-84, i.e.
+// S_aa = 3(m_a^2 - m_aa)/(2 * 4.5 tau) = (m_a^2 - m_aa) s_diag and
+// S_ab = 3(m_a m_b - m_ab)/(2 * 9 tau) = (m_a m_b - m_ab) s_off with the
+// coefficients 1/(3 tau), 1/(6 tau) formed once on the host.
+__device__ __forceinline__ void fo_values(const float m[10], float s_diag, float s_off, float &kk, float S[6]) {
     kk = 0.5f * sqrtf(m[1] * m[1] + m[2] * m[2] + m[3] * m[3]);
-    const float it = 1.0f / tau;
-    S[0] = 3.f * (m[1] * m[1] - m[4]) / 9.f * it;
-    S[1] = 3.f * (m[1] * m[2] - m[5]) / 18.f * it;
-    S[2] = 3.f * (m[1] * m[3] - m[6]) / 18.f * it;
-    S[3] = 3.f * (m[2] * m[2] - m[7]) / 9.f * it;
-    S[4] = 3.f * (m[2] * m[3] - m[8]) / 18.f * it;
-    S[5] = 3.f * (m[3] * m[3] - m[9]) / 9.f * it;
+    S[0] = (m[1] * m[1] - m[4]) * s_diag;
+    S[1] = (m[1] * m[2] - m[5]) * s_off;
+    S[2] = (m[1] * m[3] - m[6]) * s_off;
+    S[3] = (m[2] * m[2] - m[7]) * s_diag;
+    S[4] = (m[2] * m[3] - m[8]) * s_off;
+    S[5] = (m[3] * m[3] - m[9]) * s_diag;
 }
 
 // The fused step (body in step_body.inc).  PROBE replaces the moment algebra by

@@ -3,9 +3,13 @@ oracle (lid-driven cavity from rest, Re 1000, lid 0.05, 64^3, 600 steps).
 
 Fields: rho - 1, velocity (u, v, w together), the diagonal and off-diagonal
 second moments.  Bounds: FP32 velocity rel-L2 <= 1e-5 (the solver target);
-FP16 velocity rel-L2 <= 1e-3 (the storage target), and every field's max
-error below 5e-3 of that field's largest magnitude.  The table is printed for
-the record.
+FP16 velocity rel-L2 <= 1e-3 (the storage target; the stored 3u is rounded
+stochastically, which keeps this error from growing like a bias: measured
+4.9e-4 for D3Q19 and 4.5e-4 for D3Q27; a CPU emulation of the rounding with
+uniform random bits in the FP64 oracle gives 5.5e-4 at 600 steps and 6.0e-4
+at 1000, against 1.15e-3 and 1.49e-3 with round-to-nearest, which is what
+the GPU showed before), and every field's max error below 5e-3 of that
+field's largest magnitude.  The table is printed for the record.
 """
 import numpy as np
 import pytest
