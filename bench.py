@@ -47,16 +47,19 @@ def measured_traffic(key):
         return None
 
 
-def warm_vs_cold(n_cases=2, edge=128, max_steps=150000):
-    """Held-out cavities at edge^3 (Re log-uniform [400, 2500]) solved to residual
-    1e-6 from rest and from the trained surrogate's prediction."""
+def warm_vs_cold(n_draw=3, edge=128, max_steps=200000, re_steady=1500.0):
+    """Held-out cavities at edge^3 (the first n_draw of the evaluation draw, Re
+    log-uniform [400, 2500]) solved to residual 1e-6 from rest and from the
+    trained surrogate's prediction.  Only the cases with Re <= re_steady are
+    run: above it the cavity is not steady at this resolution (Re 1941 of the
+    draw keeps r ~ 1e-3 through 200k steps from either start)."""
     import numpy as np
 
     from cudalbm_amd import warmstart as W
 
     net, meta = W.load_model(MODEL_FILE)
     with net:
-        cases = W.draw_cases(n_cases, W.EVAL_SEED, (edge,), re_range=(400.0, 2500.0))
+        cases = [c for c in W.draw_cases(n_draw, W.EVAL_SEED, (edge,), re_range=(400.0, 2500.0)) if c[1] <= re_steady]
         res = W.evaluate(net, cases, max_steps=max_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
     return {"cases": [{"N": r["N"], "Re": round(r["Re"], 1), "U": round(r["U"], 4),
                        "cold_steps": r["cold"]["steps"], "cold_ms": round(r["cold"]["ms"], 1),
@@ -135,9 +138,10 @@ def cpu_baseline(target_s=25.0):
     threads = max(1, min(os.cpu_count() or 1, 16))
     om = O.omega(0.05, 1000.0, n)
     mom = O.rest_state(n, n, n)
+    O.run(19, n, n, n, om, 0.05, mom, 2, threads)  # first touch and thread start-up, untimed
     t0 = time.perf_counter()
-    O.run(19, n, n, n, om, 0.05, mom, 2, threads)
-    per = (time.perf_counter() - t0) / 2
+    O.run(19, n, n, n, om, 0.05, mom, 10, threads)
+    per = (time.perf_counter() - t0) / 10
     steps = max(2, int(target_s / max(per, 1e-6)))
     t0 = time.perf_counter()
     O.run(19, n, n, n, om, 0.05, mom, steps, threads)

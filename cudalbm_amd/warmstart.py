@@ -1,6 +1,8 @@
 """Learned warm start: data generation, training and evaluation.
 
     python -m cudalbm_amd.warmstart train --out cudalbm_amd/models/unet32.npz
+    python -m cudalbm_amd.warmstart gen --set hi --out hi.npz      (data only)
+    python -m cudalbm_amd.warmstart train --data base.npz hi.npz --out model.npz
     python -m cudalbm_amd.warmstart eval  --model cudalbm_amd/models/unet32.npz
 
 Data: lid-driven cavities (D3Q19 FP32) drawn like the batched sweep (Re
@@ -36,6 +38,12 @@ S_NET = 32
 # training edges: the sweep's range up to 128, weighted towards the cheap ones
 TRAIN_EDGES = (32, 48, 64, 96, 128)
 TRAIN_EDGE_P = (0.25, 0.25, 0.25, 0.15, 0.10)
+# data sets: "base" as above; "hi" adds cavities at the large edges, where a
+# 32^3 prediction is interpolated furthest, with a higher step cap
+DATA_SETS = {
+    "base": {"cases": 512, "seed": TRAIN_SEED, "edges": TRAIN_EDGES, "edge_p": TRAIN_EDGE_P, "max_steps": 80000},
+    "hi": {"cases": 160, "seed": TRAIN_SEED + 1, "edges": (96, 128), "edge_p": (0.4, 0.6), "max_steps": 150000},
+}
 
 
 def draw_cases(n, seed, edges, re_range=(100.0, 3200.0), u_range=(0.02, 0.08), edge_p=None):
@@ -164,6 +172,18 @@ def evaluate(net, cases, tol=1e-6, check=100, max_steps=200000, log=print):
     return out
 
 
+def save_data(path, targets, info):
+    """Targets (FP16: their values are O(1) and the fit error is ~1e-2) and case list."""
+    np.savez_compressed(path, targets=np.asarray(targets, np.float16), info=json.dumps(info))
+
+
+def load_data(path):
+    z = np.load(path, allow_pickle=False)
+    info = json.loads(str(z["info"]))
+    inputs = np.stack([N.encode_inputs(S_NET, c["Re"], c["N"]) for c in info])
+    return inputs, z["targets"].astype(np.float32), info
+
+
 def save_model(path, net, meta):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez_compressed(path, params=net.get_params(), meta=json.dumps(meta))
@@ -179,8 +199,12 @@ def load_model(path):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
+    g = sub.add_parser("gen")
+    g.add_argument("--set", choices=sorted(DATA_SETS), default="base")
+    g.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--out", required=True)
+    t.add_argument("--data", nargs="*", default=[], help="data files from `gen` (default: generate the base set)")
     t.add_argument("--cases", type=int, default=512)
     t.add_argument("--max-steps", type=int, default=80000)
     t.add_argument("--epochs", type=int, default=150)
@@ -194,15 +218,34 @@ def main(argv=None):
     e.add_argument("--json", default="")
     a = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
-    if a.cmd == "train":
-        cases = draw_cases(a.cases, TRAIN_SEED, TRAIN_EDGES, edge_p=TRAIN_EDGE_P)
+    if a.cmd == "gen":
+        d = DATA_SETS[a.set]
+        cases = draw_cases(d["cases"], d["seed"], d["edges"], edge_p=d["edge_p"])
         t0 = time.perf_counter()
-        x, y, info = generate(cases, max_steps=a.max_steps, log=log)
+        _, y, info = generate(cases, max_steps=d["max_steps"], log=log)
+        save_data(a.out, y, info)
+        print(json.dumps({"set": a.set, "cases": len(info), "generate_s": time.perf_counter() - t0,
+                          "converged": int(sum(i["residual"] < 1e-6 for i in info))}), flush=True)
+    elif a.cmd == "train":
+        t0 = time.perf_counter()
+        if a.data:
+            parts = [load_data(p) for p in a.data]
+            x = np.concatenate([p[0] for p in parts])
+            y = np.concatenate([p[1] for p in parts])
+            info = [c for p in parts for c in p[2]]
+            sets = [os.path.basename(p) for p in a.data]
+        else:
+            cases = draw_cases(a.cases, TRAIN_SEED, TRAIN_EDGES, edge_p=TRAIN_EDGE_P)
+            x, y, info = generate(cases, max_steps=a.max_steps, log=log)
+            sets = ["base"]
         gen_s = time.perf_counter() - t0
         net, losses, train_s = train(x, y, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
-        meta = {"cases": len(cases), "seed": TRAIN_SEED, "edges": TRAIN_EDGES, "generate_s": gen_s, "train_s": train_s,
-                "final_loss": float(losses[-1]), "converged": int(sum(i["residual"] < 1e-6 for i in info)),
-                "epochs": len(losses), "batch": a.batch, "lr": a.lr}
+        edges = sorted({c["N"] for c in info})
+        meta = {"cases": len(info), "data": sets, "edges": edges,
+                "per_edge": {str(e): sum(c["N"] == e for c in info) for e in edges}, "generate_s": gen_s,
+                "train_s": train_s, "final_loss": float(losses[-1]),
+                "converged": int(sum(i["residual"] < 1e-6 for i in info)), "epochs": len(losses), "batch": a.batch,
+                "lr": a.lr}
         save_model(a.out, net, meta)
         print(json.dumps(meta), flush=True)
     else:
