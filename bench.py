@@ -146,9 +146,20 @@ def cpu_baseline(target_s=25.0):
     t0 = time.perf_counter()
     O.run(19, n, n, n, om, 0.05, mom, steps, threads)
     dt = time.perf_counter() - t0
+    # one thread, a few steps, for the single-core figure
+    t1 = time.perf_counter()
+    O.run(19, n, n, n, om, 0.05, mom, 3, 1)
+    single = n ** 3 * 3 / (time.perf_counter() - t1) / 1e6
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle, D3Q19 {n}^3 lid cavity Re 1000 u 0.05 from rest, {steps} steps, "
-                      f"{dt:.1f} s on {threads} threads"}
+                      f"{dt:.1f} s on {threads} threads of {os.cpu_count()} ({model}); "
+                      f"one thread: {single:.1f} MLUPS"}
 
 
 def main():
@@ -201,11 +212,17 @@ def main():
     variants = {}
     if not args.no_variants:
         k = max(50, args.steps // 4)
-        for name, (Q, st, fo) in {"d3q19_fp32_step_only": (19, "fp32", 0), "d3q19_fp16_step_only": (19, "fp16", 0),
-                                  "d3q19_fp16_fo": (19, "fp16", FO_ALL), "d3q27_fp32_step_only": (27, "fp32", 0),
-                                  "d3q27_fp16_step_only": (27, "fp16", 0)}.items():
-            r = measure(args.n, Q, st, k, 10, fo_mask=fo)
-            variants[name] = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+        n = args.n
+        for name, (nv, Q, st, fo) in {"d3q19_fp32_step_only": (n, 19, "fp32", 0),
+                                      "d3q19_fp16_step_only": (n, 19, "fp16", 0),
+                                      "d3q19_fp16_fo": (n, 19, "fp16", FO_ALL),
+                                      "d3q27_fp32_step_only": (n, 27, "fp32", 0),
+                                      "d3q27_fp16_step_only": (n, 27, "fp16", 0),
+                                      # the survey's larger D3Q27 mesh (2.3 GB of moments)
+                                      "d3q27_fp32_step_only_384": (384, 27, "fp32", 0),
+                                      "d3q27_fp16_step_only_384": (384, 27, "fp16", 0)}.items():
+            r = measure(nv, Q, st, k, 10, fo_mask=fo)
+            variants[name] = {"n": nv, "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
                               "roofline_frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4),
                               "bytes_per_node": r["bytes_per_node"], "tiling": r["tiling"]}
             print(f"# {name}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)

@@ -236,9 +236,9 @@ template <int Q>
 __device__ __forceinline__ int zf_off(const StepParams &p, int s, int c, int k, int y, int x) {
     return p.zoff + (((s * p.nc + c) * VS<Q>::QF + k) * p.ny + y) * p.nx + x;
 }
-// Population q of node (x,y,z) rebuilt from its moments (or its probe id).
+// Population q of node (x,y,z) from its rebuild terms (or its probe id).
 template <int Q, int q, bool PROBE>
-__device__ __forceinline__ float pop_of(const Rec &r, const float (&m)[10], int x, int y, int z,
+__device__ __forceinline__ float pop_of(const Rec &r, int x, int y, int z,
                                         const StepParams &p) {
     if constexpr (PROBE)
         return probe_id(x, y, z, p.nx, p.ny, q);
@@ -257,25 +257,25 @@ __device__ __forceinline__ void write_xy_faces(const StepParams &p, float *__res
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             constexpr int q = face_q<Q>(0, -1, k);
-            fout[xf_off<Q>(p, 0, bx, k, z, y)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+            fout[xf_off<Q>(p, 0, bx, k, z, y)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
     if (tx == TX - 1 && bx < p.nbx - 1)
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             constexpr int q = face_q<Q>(0, 1, k);
-            fout[xf_off<Q>(p, 1, bx, k, z, y)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+            fout[xf_off<Q>(p, 1, bx, k, z, y)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
     if (ty == 0 && by > 0)
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             constexpr int q = face_q<Q>(1, -1, k);
-            fout[yf_off<Q>(p, 0, by, k, z, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+            fout[yf_off<Q>(p, 0, by, k, z, x)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
     if (ty == TY - 1 && by < p.nby - 1)
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             constexpr int q = face_q<Q>(1, 1, k);
-            fout[yf_off<Q>(p, 1, by, k, z, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+            fout[yf_off<Q>(p, 1, by, k, z, x)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
 }
 
@@ -291,13 +291,13 @@ __device__ __forceinline__ void write_z_faces(const StepParams &p, float *__rest
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             constexpr int q = face_q<Q>(2, -1, k);
-            fout[zf_off<Q>(p, 0, c, k, y, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+            fout[zf_off<Q>(p, 0, c, k, y, x)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
     if (z == z1 && c < p.nc - 1)
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             constexpr int q = face_q<Q>(2, 1, k);
-            fout[zf_off<Q>(p, 1, c, k, y, x)] = pop_of<Q, q, PROBE>(r, m, x, y, z, p);
+            fout[zf_off<Q>(p, 1, c, k, y, x)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
 }
 

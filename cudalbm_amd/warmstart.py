@@ -89,8 +89,9 @@ def normalised_fields(mom, N, U, S=S_NET):
     return np.stack(cols, -1).astype(np.float32)
 
 
-def generate(cases, tol=1e-6, check=100, max_steps=60000, log=print):
-    """Solve cases [(N, Re, U)] with one batch per edge; returns (inputs, targets, info)."""
+def generate(cases, tol=1e-6, check=100, max_steps=60000, log=print, S=S_NET):
+    """Solve cases [(N, Re, U)] with one batch per edge; returns (inputs, targets, info)
+    with inputs and targets on the network's S^3 grid."""
     inputs, targets, info = [], [], []
     by_edge = {}
     for i, c in enumerate(cases):
@@ -121,16 +122,21 @@ def generate(cases, tol=1e-6, check=100, max_steps=60000, log=print):
         if not (np.isfinite(res) and all(np.all(np.isfinite(m)) for m in mom)):
             dropped += 1  # a case the scheme cannot hold at this resolution
             continue
-        inputs.append(N.encode_inputs(S_NET, Re, n))
-        targets.append(normalised_fields(mom, n, U))
+        inputs.append(N.encode_inputs(S, Re, n))
+        targets.append(normalised_fields(mom, n, U, S))
         info.append({"N": n, "Re": Re, "U": U, "steps": steps, "residual": res})
     if dropped:
         log(f"# dropped {dropped} non-finite cases")
     return np.stack(inputs), np.stack(targets), info
 
 
-def train(inputs, targets, epochs=150, batch=8, lr=1e-3, seed=TRAIN_SEED, log=print):
-    net = N.Surrogate(S=S_NET, seed=seed)
+def train(inputs, targets, epochs=150, batch=8, lr=1e-3, seed=TRAIN_SEED, log=print, S=S_NET, init=None):
+    """Adam in three learning-rate stages; `init`: parameters to start from (the
+    network is fully convolutional, so weights trained on one grid size are a
+    starting point on another)."""
+    net = N.Surrogate(S=S, seed=seed)
+    if init is not None:
+        net.set_params(init)
     net.load_data(inputs, targets)
     t0 = time.perf_counter()
     losses = []
@@ -172,16 +178,17 @@ def evaluate(net, cases, tol=1e-6, check=100, max_steps=200000, log=print):
     return out
 
 
-def save_data(path, targets, info):
+def save_data(path, targets, info, S=S_NET):
     """Targets (FP16: their values are O(1) and the fit error is ~1e-2) and case list."""
-    np.savez_compressed(path, targets=np.asarray(targets, np.float16), info=json.dumps(info))
+    np.savez_compressed(path, targets=np.asarray(targets, np.float16), info=json.dumps(info), S=S)
 
 
 def load_data(path):
     z = np.load(path, allow_pickle=False)
     info = json.loads(str(z["info"]))
-    inputs = np.stack([N.encode_inputs(S_NET, c["Re"], c["N"]) for c in info])
-    return inputs, z["targets"].astype(np.float32), info
+    S = int(z["S"]) if "S" in z else S_NET
+    inputs = np.stack([N.encode_inputs(S, c["Re"], c["N"]) for c in info])
+    return inputs, z["targets"].astype(np.float32), info, S
 
 
 def save_model(path, net, meta):
@@ -191,9 +198,10 @@ def save_model(path, net, meta):
 
 def load_model(path):
     z = np.load(path, allow_pickle=False)
-    net = N.Surrogate(S=S_NET, seed=0)
+    meta = json.loads(str(z["meta"]))
+    net = N.Surrogate(S=int(meta.get("S", S_NET)), seed=0)
     net.set_params(z["params"])
-    return net, json.loads(str(z["meta"]))
+    return net, meta
 
 
 def main(argv=None):
@@ -201,10 +209,15 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     g = sub.add_parser("gen")
     g.add_argument("--set", choices=sorted(DATA_SETS), default="base")
+    g.add_argument("--min-edge", type=int, default=0, help="keep only the set's cases with N >= this")
+    g.add_argument("--max-edge", type=int, default=1 << 30, help="keep only the set's cases with N <= this")
+    g.add_argument("--S", type=int, default=S_NET, help="network grid edge of the targets")
     g.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--out", required=True)
     t.add_argument("--data", nargs="*", default=[], help="data files from `gen` (default: generate the base set)")
+    t.add_argument("--S", type=int, default=S_NET, help="network grid edge (must match the data files)")
+    t.add_argument("--init", default="", help="model file whose parameters start the training")
     t.add_argument("--cases", type=int, default=512)
     t.add_argument("--max-steps", type=int, default=80000)
     t.add_argument("--epochs", type=int, default=150)
@@ -220,28 +233,36 @@ def main(argv=None):
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "gen":
         d = DATA_SETS[a.set]
-        cases = draw_cases(d["cases"], d["seed"], d["edges"], edge_p=d["edge_p"])
+        cases = [c for c in draw_cases(d["cases"], d["seed"], d["edges"], edge_p=d["edge_p"])
+                 if a.min_edge <= c[0] <= a.max_edge]
         t0 = time.perf_counter()
-        _, y, info = generate(cases, max_steps=d["max_steps"], log=log)
-        save_data(a.out, y, info)
-        print(json.dumps({"set": a.set, "cases": len(info), "generate_s": time.perf_counter() - t0,
+        _, y, info = generate(cases, max_steps=d["max_steps"], log=log, S=a.S)
+        save_data(a.out, y, info, S=a.S)
+        print(json.dumps({"set": a.set, "S": a.S, "cases": len(info), "generate_s": time.perf_counter() - t0,
                           "converged": int(sum(i["residual"] < 1e-6 for i in info))}), flush=True)
     elif a.cmd == "train":
         t0 = time.perf_counter()
         if a.data:
             parts = [load_data(p) for p in a.data]
+            if any(p[3] != a.S for p in parts):
+                raise SystemExit(f"data files were generated for S = {[p[3] for p in parts]}, not --S {a.S}")
             x = np.concatenate([p[0] for p in parts])
             y = np.concatenate([p[1] for p in parts])
             info = [c for p in parts for c in p[2]]
             sets = [os.path.basename(p) for p in a.data]
         else:
             cases = draw_cases(a.cases, TRAIN_SEED, TRAIN_EDGES, edge_p=TRAIN_EDGE_P)
-            x, y, info = generate(cases, max_steps=a.max_steps, log=log)
+            x, y, info = generate(cases, max_steps=a.max_steps, log=log, S=a.S)
             sets = ["base"]
         gen_s = time.perf_counter() - t0
-        net, losses, train_s = train(x, y, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+        init = None
+        if a.init:
+            init_net, init_meta = load_model(a.init)
+            with init_net:
+                init = init_net.get_params()
+        net, losses, train_s = train(x, y, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, S=a.S, init=init)
         edges = sorted({c["N"] for c in info})
-        meta = {"cases": len(info), "data": sets, "edges": edges,
+        meta = {"S": a.S, "cases": len(info), "data": sets, "edges": edges, "init": os.path.basename(a.init),
                 "per_edge": {str(e): sum(c["N"] == e for c in info) for e in edges}, "generate_s": gen_s,
                 "train_s": train_s, "final_loss": float(losses[-1]),
                 "converged": int(sum(i["residual"] < 1e-6 for i in info)), "epochs": len(losses), "batch": a.batch,

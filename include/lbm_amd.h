@@ -75,7 +75,8 @@ typedef struct lbm_config {
 
 /* Replaces host::latticeMesh + device::array + device::halo construction
  * (momentBasedD3Q19.cu:66-98).  Allocates the moments (initialised to rest)
- * and both face-buffer sets. */
+ * and both face-buffer sets.  At most 2^30 nodes (the kernels address each
+ * moment array with 32-bit byte offsets); LBM_ERR_SIZE otherwise. */
 int lbm_create(const lbm_config *cfg, lbm_handle **out);
 int lbm_destroy(lbm_handle *h);
 const char *lbm_last_error(const lbm_handle *h);

@@ -16,7 +16,10 @@
  *   boundary moments ..... boundaryConditions.cuh:99-554
  *                          pinned: rest fixed point at every node class (exact
  *                          rationals 1 and 125/126), per-case algebra written
- *                          independently in the tests, and mass/velocity checks
+ *                          independently in the tests, and mass/velocity checks;
+ *                          parity unpinned for the U-dependent terms of the
+ *                          NORTH-WEST/EAST lid edges, the lid corners and the
+ *                          D3Q27 NORTH-BACK/FRONT edges (transcription only)
  *   scale / collide ...... velocitySet.cuh:130-142, secondOrder.cuh:93-105
  *                          pinned: affine closed form + equilibrium fixed point
  *   node type ............ normalVector.cuh:321-334 (pinned: exhaustive counts)

@@ -110,3 +110,61 @@ def test_parity_tilings_d3q19(native_lib, tile):
     errs = _parity(19, 32, 400.0, 0.05, 200, tile=tile)
     assert errs["velocity"] <= 1e-5
     assert errs["rho"] <= 1e-5
+
+
+@pytest.mark.parametrize("Q,storage,tiles", [
+    (19, "fp32", [(32, 16, 32), (16, 16, 8), (8, 8, 4)]),
+    (19, "fp16", [(32, 16, 32), (16, 16, 8), (8, 8, 4)]),
+    (27, "fp32", [(16, 16, 32), (8, 8, 4)]),
+    (27, "fp16", [(16, 16, 32), (8, 8, 4)]),
+])
+def test_determinism_runs_and_tilings(native_lib, Q, storage, tiles):
+    """A repeat run at the same tiling is bit-identical.  Across tilings the
+    kernels are different instantiations whose FMA contraction may differ, so
+    FP32 results agree to round-off (max |diff| <= 1e-6); with FP16 storage a
+    one-ulp FP32 difference can flip a stochastic-rounding decision, so there
+    the velocity fields agree to the storage tolerance (rel-L2 <= 1e-3) -- the
+    random words themselves come from global coordinates and the step, not
+    from the tiling."""
+    from cudalbm_amd import Solver
+
+    n, steps = 32, 150
+
+    def run(tile):
+        with Solver(n, n, n, 400.0, 0.05, velocity_set=Q, storage=storage, tile=tile) as s:
+            s.step(steps)
+            return np.stack(s.get_moments())
+
+    ref = run(tiles[0])
+    assert np.array_equal(run(tiles[0]), ref), "repeat run differs"
+    for tile in tiles[1:]:
+        got = run(tile)
+        dmax = float(np.abs(got - ref).max())
+        rel = rel_l2(got[1:4].astype(np.float64), ref[1:4].astype(np.float64))
+        print(f"Q{Q} {storage} tile={tile} vs {tiles[0]}: max |diff| {dmax:.3e}, velocity rel-L2 {rel:.3e}",
+              flush=True)
+        if storage == "fp32":
+            assert dmax <= 1e-6, (tile, dmax)
+        else:
+            assert rel <= 1e-3, (tile, rel)
+
+
+def test_parity_d3q27_128_re1000_fp32_fp16(native_lib):
+    """The D3Q27 case at the oracle-checkable mesh of the 256^3/384^3 D3Q27
+    configuration: 128^3, Re 1000, lid 0.05, 150 steps from rest.  FP32:
+    velocity and density rel-L2 <= 1e-5; FP16 storage: velocity rel-L2 <= 1e-3,
+    density <= 1e-4."""
+    from cudalbm_amd import Solver
+
+    n, Re, u, steps = 128, 1000.0, 0.05, 150
+    ref = O.run(27, n, n, n, O.omega(u, Re, n), u, O.rest_state(n, n, n), steps)
+    vel_ref = np.stack(ref[1:4])
+    for storage, bound, bound_rho in (("fp32", 1e-5, 1e-5), ("fp16", 1e-3, 1e-4)):
+        with Solver(n, n, n, Re, u, velocity_set=27, storage=storage) as s:
+            s.step(steps)
+            got = s.get_moments()
+        ev = rel_l2(np.stack(got[1:4]).astype(np.float64), vel_ref)
+        er = rel_l2(1.0 + got[0].astype(np.float64), 1.0 + ref[0])
+        print(f"Q27 128^3 Re 1000 {steps} steps {storage}: velocity {ev:.3e} rho {er:.3e}", flush=True)
+        assert ev <= bound
+        assert er <= bound_rho

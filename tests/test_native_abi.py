@@ -54,3 +54,8 @@ def test_create_rejects_bad_config_without_gpu_work(native_lib):
     cfg.nx = 20  # not a multiple of any footprint with ny=16
     cfg.tile_x, cfg.tile_y = 16, 16
     assert native_lib.lbm_create(ctypes.byref(cfg), ctypes.byref(h)) == -49
+    # moment arrays are addressed with 32-bit byte offsets: at most 2^30 nodes
+    cfg.nx = cfg.ny = cfg.nz = 1056
+    cfg.tile_x = cfg.tile_y = cfg.tile_z = 0
+    assert native_lib.lbm_create(ctypes.byref(cfg), ctypes.byref(h)) == -49
+    assert b"2^30" in native_lib.lbm_create_error()

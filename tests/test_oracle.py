@@ -219,16 +219,27 @@ def _closure_by_matching(Q, bits, f, omega, u_lid):
     return xv[0], dict(zip([k for _, _, k in active], xv[1:]))
 
 
-def _static_or_north_face(bits):
+def _lid_velocity_in_derivation(Q, bits, u_lid):
+    """Wall velocity with which the matching derivation reproduces the closure:
+    static walls 0; the NORTH face the lid speed, and for D3Q19 also the
+    NORTH-BACK / NORTH-FRONT edges (their stored velocity is the
+    boundary-count average U/2, but their closure is the moving face's); None
+    where the reference's closure follows another derivation (NORTH-WEST/EAST
+    edges, lid corners, and the D3Q27 lid edges)."""
     nb = bin(bits & 0x3F).count("1")
-    return (not (bits & 8)) or nb == 1
+    if not (bits & 8):
+        return 0.0
+    if nb == 1 or (Q == 19 and (bits & 0x3F) in (8 | 16, 8 | 32)):
+        return u_lid
+    return None
 
 
 @pytest.mark.parametrize("Q", [19, 27])
 def test_boundary_closure_matches_independent_derivation(Q):
-    """All static faces, edges and corners and the moving NORTH face agree with the
-    matching derivation to 1e-10.  (For D3Q27 the static corners use 216/125; the
-    derivation gives the same.)  The lid edges and corners carry U-dependent
+    """All static faces, edges and corners, the moving NORTH face and (D3Q19)
+    the NORTH-BACK / NORTH-FRONT lid edges agree with the matching derivation to
+    1e-10.  (For D3Q27 the static corners use 216/125; the derivation gives the
+    same.)  The other lid edges and the four lid corners carry U-dependent
     terms that the reference derived differently; they are pinned by their U=0
     limit here and by the transcription (oracle header)."""
     rng = np.random.default_rng(3)
@@ -247,7 +258,8 @@ def test_boundary_closure_matches_independent_derivation(Q):
         for q in range(Q):
             if not O.known(Q, q, bits):
                 f[q] = 0.0
-        U = u_lid if _static_or_north_face(bits) else 0.0
+        U = _lid_velocity_in_derivation(Q, bits, u_lid)
+        U = 0.0 if U is None else U  # the U = 0 limit of the other lid nodes
         if Q == 27 and (bits & 8) and bin(bits & 0x3F).count("1") == 3:
             # D3Q27 North corners reuse the D3Q19 constant (reference quirk,
             # boundaryConditions.cuh:419-466); pinned by the exact 125/126 test
