@@ -34,7 +34,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X published HBM3E bandwidth
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-MODEL_FILE = os.path.join(ROOT, "cudalbm_amd", "models", "unet32.npz")
+# warm-start network: the 64^3-grid model where present (closer to the 128^3
+# meshes of the warm-vs-cold leg), else the 32^3 one
+MODEL_FILE = next((p for p in (os.path.join(ROOT, "cudalbm_amd", "models", f) for f in ("unet64.npz", "unet32.npz"))
+                   if os.path.exists(p)), os.path.join(ROOT, "cudalbm_amd", "models", "unet32.npz"))
 
 
 def measured_traffic(key):
@@ -69,7 +72,8 @@ def warm_vs_cold(n_draw=3, edge=128, max_steps=200000, re_steady=1500.0):
                        "final_velocity_rel_l2": r["final_rel_l2"]} for r in res],
             "median_time_ratio": float(np.median([r["time_ratio"] for r in res])),
             "max_final_rel_l2": float(max(r["final_rel_l2"] for r in res)),
-            "model": {k: meta[k] for k in ("cases", "epochs", "final_loss") if k in meta}}
+            "model": {"file": os.path.basename(MODEL_FILE),
+                      **{k: meta[k] for k in ("S", "cases", "epochs", "final_loss") if k in meta}}}
 
 
 def measure(n, Q, storage, steps, warmup, fo_mask=0, Re=1000.0, u=0.05, tile=(0, 0, 0)):

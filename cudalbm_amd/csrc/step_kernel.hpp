@@ -248,11 +248,9 @@ __device__ __forceinline__ float pop_of(const Rec &r, int x, int y, int z,
 
 // Outgoing x/y face populations of one node after collision (or its probe ids).
 template <int Q, int TX, int TY, bool PROBE>
-__device__ __forceinline__ void write_xy_faces(const StepParams &p, float *__restrict__ fout,
-                                               const float (&m)[10], int tx, int ty, int x, int y,
-                                               int z, int bx, int by) {
+__device__ __forceinline__ void write_x_faces(const StepParams &p, float *__restrict__ fout, const Rec &r, int tx,
+                                              int x, int y, int z, int bx) {
     constexpr int QF = VS<Q>::QF;
-    const Rec r = rec_setup(m);
     if (tx == 0 && bx > 0)
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
@@ -265,6 +263,11 @@ __device__ __forceinline__ void write_xy_faces(const StepParams &p, float *__res
             constexpr int q = face_q<Q>(0, 1, k);
             fout[xf_off<Q>(p, 1, bx, k, z, y)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
+}
+template <int Q, int TX, int TY, bool PROBE>
+__device__ __forceinline__ void write_y_faces(const StepParams &p, float *__restrict__ fout, const Rec &r, int ty,
+                                              int x, int y, int z, int by) {
+    constexpr int QF = VS<Q>::QF;
     if (ty == 0 && by > 0)
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
@@ -277,6 +280,14 @@ __device__ __forceinline__ void write_xy_faces(const StepParams &p, float *__res
             constexpr int q = face_q<Q>(1, 1, k);
             fout[yf_off<Q>(p, 1, by, k, z, x)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
+}
+template <int Q, int TX, int TY, bool PROBE>
+__device__ __forceinline__ void write_xy_faces(const StepParams &p, float *__restrict__ fout,
+                                               const float (&m)[10], int tx, int ty, int x, int y,
+                                               int z, int bx, int by) {
+    const Rec r = rec_setup(m);
+    write_x_faces<Q, TX, TY, PROBE>(p, fout, r, tx, x, y, z, bx);
+    write_y_faces<Q, TX, TY, PROBE>(p, fout, r, ty, x, y, z, by);
 }
 
 // Outgoing z face populations at the chunk's first and last planes.
@@ -313,26 +324,30 @@ __device__ __forceinline__ void write_faces(const StepParams &p, float *__restri
 // numbered bottom row, top row, left column, right column (corners once).
 template <int TX, int TY>
 struct Edge {
-    static constexpr int NE = 2 * TX + 2 * (TY - 2);
-    __device__ static __forceinline__ int index(int tx, int ty) {
-        if (ty == 0) return tx;
-        if (ty == TY - 1) return TX + tx;
-        if (tx == 0) return 2 * TX + ty - 1;
-        if (tx == TX - 1) return 2 * TX + (TY - 2) + ty - 1;
-        return -1;
+    // entries: the S row (ty = 0) and the N row (ty = TY-1), then the W column
+    // (tx = 0) and the E column (tx = TX-1), corners in both lists, so that a
+    // row entry writes only y faces and a column entry only x faces
+    static constexpr int NE = 2 * TX + 2 * TY;
+    __device__ static __forceinline__ int row(int tx, int ty) {
+        return ty == 0 ? tx : (ty == TY - 1 ? TX + tx : -1);
+    }
+    __device__ static __forceinline__ int col(int tx, int ty) {
+        return tx == 0 ? 2 * TX + ty : (tx == TX - 1 ? 2 * TX + TY + ty : -1);
     }
     __device__ static __forceinline__ void coords(int e, int &tx, int &ty) {
         if (e < TX) { tx = e; ty = 0; }
         else if (e < 2 * TX) { tx = e - TX; ty = TY - 1; }
-        else if (e < 2 * TX + TY - 2) { tx = 0; ty = e - 2 * TX + 1; }
-        else { tx = TX - 1; ty = e - 2 * TX - (TY - 2) + 1; }
+        else if (e < 2 * TX + TY) { tx = 0; ty = e - 2 * TX; }
+        else { tx = TX - 1; ty = e - 2 * TX - TY; }
     }
 };
 
 // x/y face populations of one finished plane, written cooperatively: edge
 // nodes park their post-collision moments in LDS (edge_m, 10 x NE words) and
-// the first NE threads -- one or two whole waves -- rebuild and store the face
-// populations, instead of every wave rebuilding them for its few edge lanes.
+// the first NE threads -- whole waves where TX is a multiple of 32 -- rebuild
+// and store the face populations, instead of every wave rebuilding them for
+// its few edge lanes.  Row entries write the y faces, column entries the x
+// faces, so a wave of row entries runs only the y-face branches.
 template <int Q, int TX, int TY, bool PROBE>
 __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__restrict__ fout,
                                                const float *edge_m, int tid, int bx, int by, int z) {
@@ -343,7 +358,12 @@ __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__res
     float m[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) m[k] = edge_m[k * E::NE + tid];
-    write_xy_faces<Q, TX, TY, PROBE>(p, fout, m, tx, ty, bx * TX + tx, by * TY + ty, z, bx, by);
+    const Rec r = rec_setup(m);
+    const int x = bx * TX + tx, y = by * TY + ty;
+    if (tid < 2 * TX)
+        write_y_faces<Q, TX, TY, PROBE>(p, fout, r, ty, x, y, z, by);
+    else
+        write_x_faces<Q, TX, TY, PROBE>(p, fout, r, tx, x, y, z, bx);
 }
 
 // First node of this workgroup's tile in plane z (uniform); add the thread index.

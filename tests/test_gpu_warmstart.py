@@ -1,6 +1,7 @@
-"""What the trained surrogate gains: a held-out cavity solved to residual 1e-6
-from rest and from the network's prediction (the committed model
-cudalbm_amd/models/unet32.npz), plus the fallback rule on a broken network.
+"""What the trained surrogates gain: a held-out cavity solved to residual 1e-6
+from rest and from the network's prediction (the committed models
+cudalbm_amd/models/unet32.npz and unet64.npz, the 32^3 and 64^3 network
+grids), plus the fallback rule on a broken network.
 
 Bounds: the warm start takes <= 0.7x the cold steps (measured ~0.5x at 64^3)
 and both final velocity fields agree to rel-L2 <= 5e-4 (two runs stopped at
@@ -16,7 +17,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-MODEL = os.path.join(ROOT, "cudalbm_amd", "models", "unet32.npz")
+MODELS = [os.path.join(ROOT, "cudalbm_amd", "models", f) for f in ("unet32.npz", "unet64.npz")]
 CASE = (64, 600.0, 0.05)  # N, Re, lid speed
 
 
@@ -38,15 +39,18 @@ def solve(net=None):
         return info, s.get_moments()
 
 
-@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained model in the tree")
-def test_trained_warm_start_beats_cold(native_lib):
+@pytest.mark.parametrize("model", MODELS, ids=lambda p: os.path.basename(p))
+def test_trained_warm_start_beats_cold(native_lib, model):
     from cudalbm_amd import warmstart as W
 
+    if not os.path.exists(model):
+        pytest.skip("no trained model in the tree")
     cold, mc = solve()
-    net, meta = W.load_model(MODEL)
+    net, meta = W.load_model(model)
     with net:
         warm, mw = solve(net)
-    print(f"cold {cold['steps']} steps {cold['ms']:.0f} ms; warm {warm['steps']} steps {warm['ms']:.0f} ms "
+    print(f"{os.path.basename(model)} (S {net.S}): cold {cold['steps']} steps {cold['ms']:.0f} ms; "
+          f"warm {warm['steps']} steps {warm['ms']:.0f} ms "
           f"(first residual {warm['first_residual']:.3f}); final velocity rel-L2 {rel(vel(mw), vel(mc)):.2e}",
           flush=True)
     assert cold["residual"] < 1e-6 and warm["residual"] < 1e-6
