@@ -222,7 +222,7 @@ def main():
                                       "d3q19_fp16_fo": (n, 19, "fp16", FO_ALL),
                                       "d3q27_fp32_step_only": (n, 27, "fp32", 0),
                                       "d3q27_fp16_step_only": (n, 27, "fp16", 0),
-                                      # the survey's larger D3Q27 mesh (2.3 GB of moments)
+                                      # the larger D3Q27 mesh (2.3 GB of moments)
                                       "d3q27_fp32_step_only_384": (384, 27, "fp32", 0),
                                       "d3q27_fp16_step_only_384": (384, 27, "fp16", 0)}.items():
             r = measure(nv, Q, st, k, 10, fo_mask=fo)

@@ -9,7 +9,7 @@
 //                 [-fileType <vtu|vts|tecplot>] [-latestTime] [-scheme <reference|central>]
 //                 (applications/fieldCalculate/fieldCalculate.cu:60-200)
 // Output files go to postProcess/ (src/postProcess/postProcess.cuh:61).
-// -scheme is this build's addition: the reference's derivative stencils are
+// -scheme is an addition of this tool: the reference's derivative stencils are
 // reproduced by default; "central" selects the consistent central differences.
 #include <cmath>
 #include <cstdio>
