@@ -228,6 +228,8 @@ def main(argv=None):
     e.add_argument("--edges", default="128")
     e.add_argument("--cases", type=int, default=4)
     e.add_argument("--max-steps", type=int, default=200000)
+    e.add_argument("--seed", type=int, default=EVAL_SEED)
+    e.add_argument("--re-range", default="400,2500", help="Re interval of the log-uniform draw")
     e.add_argument("--json", default="")
     a = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
@@ -272,7 +274,8 @@ def main(argv=None):
     else:
         net, meta = load_model(a.model)
         edges = tuple(int(v) for v in a.edges.split(","))
-        cases = draw_cases(a.cases, EVAL_SEED, edges, re_range=(400.0, 2500.0))
+        lo, hi = (float(v) for v in a.re_range.split(","))
+        cases = draw_cases(a.cases, a.seed, edges, re_range=(lo, hi))
         res = evaluate(net, cases, max_steps=a.max_steps, log=log)
         summary = {"cases": len(res),
                    "median_time_ratio": float(np.median([r["time_ratio"] for r in res])),
