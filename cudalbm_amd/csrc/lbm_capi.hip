@@ -106,18 +106,48 @@ int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_
     return xw + yw + zw;
 }
 
-int choose_lz(int nx, int ny, int nz, int tx, int ty) {
+// Planes per work item.  A launch runs in rounds of `slots` resident
+// workgroups; a work item of lz planes costs about (lz + 4) plane times (the
+// 4: pipeline fill and drain, ring and z-face set-up) at bpn(lz) bytes per
+// node, bpn = 80 + 16 QF (1/TX + 1/TY + 1/lz) with the face words.  The
+// choice minimises rounds x (lz + 4) x bpn(lz) over the divisors of nz in
+// [4, 64], with a 10% penalty for a single round (no other workgroup overlaps
+// the first and last planes).  Measured on MI355X (MLUPS, best choice = the
+// model's): D3Q19 32x16 at 128^3 lz 8 (45.0k; lz 4: 40.9k, 16: 40.5k), 192^3
+// lz 16 (52.8k; 24: 43.7k), 256^3 lz 32 (63.0k; 64: 61.6k); D3Q27 16x16 at
+// 128^3 lz 16 (31.4k; 8: 27.7k), 192^3 lz 12 (40.0k; 16: 37.8k), 256^3 lz 32
+// (42.6k; 64: 38.4k), 384^3 lz 48 (45.1k; 64: 44.0k).
+int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots) {
     const int64_t cols = (int64_t)(nx / tx) * (ny / ty);
+    const double face = 16.0 * (Q == 19 ? 5 : 9);
     int best = -1;
-    for (int lz = std::min(nz, 64); lz >= 1; lz--) {
+    double best_cost = 0.0;
+    for (int lz = 4; lz <= std::min(nz, 64); lz++) {
         if (nz % lz) continue;
-        if (cols * (nz / lz) >= 1024) return lz;
-        best = lz;  // smallest divisor seen so far
+        const int64_t items = cols * (nz / lz);
+        const int64_t rounds = (items + slots - 1) / slots;
+        const double bpn = 80.0 + face * (1.0 / tx + 1.0 / ty + 1.0 / lz);
+        const double cost = (double)rounds * (lz + 4) * bpn * (rounds == 1 ? 1.1 : 1.0);
+        if (best < 0 || cost < best_cost) {
+            best = lz;
+            best_cost = cost;
+        }
     }
-    // not enough columns for 1024 workgroups: keep chunks of at least 4 planes
-    for (int lz = 4; lz <= nz; lz++)
+    if (best > 0) return best;
+    for (int lz = std::min(nz, 64); lz >= 1; lz--)  // nz has no divisor in [4, 64]
         if (nz % lz == 0) return lz;
-    return best > 0 ? best : nz;
+    return nz;
+}
+
+// Workgroups resident at once: CUs x workgroups per CU at the step kernel's
+// occupancy (4 waves per SIMD for D3Q19, 3 for D3Q27; 4 SIMDs per CU).
+int64_t resident_workgroups(int Q, int tx, int ty, int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        cus = 256;  // MI355X
+    const int waves = (tx * ty + 63) / 64;
+    const int per_cu = std::max(1, (4 * (Q == 19 ? 4 : 3)) / waves);
+    return (int64_t)cus * per_cu;
 }
 
 void choose_tile(int Q, int nx, int ny, int *tx, int *ty) {
@@ -456,14 +486,16 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     if (c.nx % h->TX || c.ny % h->TY)
         return bad(LBM_ERR_SIZE, "nx and ny must be multiples of the tile footprint " + std::to_string(h->TX) +
                                      "x" + std::to_string(h->TY));
-    h->LZ = c.tile_z > 0 ? c.tile_z : choose_lz(c.nx, c.ny, c.nz, h->TX, h->TY);
+    h->n = (int64_t)c.nx * c.ny * c.nz;
+    // moment arrays are addressed with 32-bit byte offsets (step_kernel.hpp, Store)
+    if (h->n > ((int64_t)1 << 30)) return bad(LBM_ERR_SIZE, "mesh too large: at most 2^30 nodes (4 GiB per moment array)");
+    h->LZ = c.tile_z > 0 ? c.tile_z
+                         : choose_lz(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY,
+                                     resident_workgroups(h->Q, h->TX, h->TY, c.device));
     if (c.nz % h->LZ) return bad(LBM_ERR_SIZE, "nz must be a multiple of tile_z");
     h->nbx = c.nx / h->TX;
     h->nby = c.ny / h->TY;
     h->nc = c.nz / h->LZ;
-    h->n = (int64_t)c.nx * c.ny * c.nz;
-    // moment arrays are addressed with 32-bit byte offsets (step_kernel.hpp, Store)
-    if (h->n > ((int64_t)1 << 30)) return bad(LBM_ERR_SIZE, "mesh too large: at most 2^30 nodes (4 GiB per moment array)");
     relaxation(c.Re, c.u_inf, c.nx, &h->omega, &h->tau);
     try {
         HIPX(hipSetDevice(c.device));
