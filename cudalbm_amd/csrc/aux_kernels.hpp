@@ -18,15 +18,6 @@ static __global__ void decode_f16_kernel(const _Float16 *__restrict__ src, float
         dst[i] = float(src[i]) * dec;
 }
 
-// Advances the step counter read by the fused time averages.
-static __global__ void tick_kernel(StepClock *c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const int64_t t = c->t + 1;
-        c->t = t;
-        c->inv = 1.0 / double(t + 1);
-    }
-}
-
 static __global__ void fill_kernel_f64(double *p, int64_t n, double v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;

@@ -245,7 +245,8 @@ struct lbm_handle {
         p.fo_mask = probe ? 0u : fo_mask;
         p.kmean = kmean;
         p.smean = smean;
-        p.clock = clock;
+        p.clock = clock + parity;
+        p.clock_next = clock + (parity ^ 1);
         p.s_diag = (float)(1.0 / (3.0 * tau));
         p.s_off = (float)(1.0 / (6.0 * tau));
         p.n = n;
@@ -389,12 +390,13 @@ void check_layout(lbm_handle *h, int layout) {
         throw ArgError{LBM_ERR_SIZE, "TILED8 layout needs nx, ny, nz multiples of 8"};
 }
 
-// Device step clock <- the handle's time t (stream-ordered; the caller syncs).
+// Device step clock of the next step (the one of the current face parity) <-
+// the handle's time t (stream-ordered; the caller syncs).
 void set_clock(lbm_handle *h) {
     static thread_local StepClock c;  // stays valid until the copy has run
     c.t = h->t;
     c.inv = 1.0 / double(h->t + 1);
-    HIPX(hipMemcpyAsync(h->clock, &c, sizeof c, hipMemcpyHostToDevice, h->stream));
+    HIPX(hipMemcpyAsync(h->clock + h->parity, &c, sizeof c, hipMemcpyHostToDevice, h->stream));
     HIPX(hipStreamSynchronize(h->stream));
 }
 
@@ -403,12 +405,9 @@ void enqueue_step(lbm_handle *h, int parity) {
     StepParams p = h->params();
     p.fin = h->faces[parity];
     p.fout = h->faces[parity ^ 1];
+    p.clock = h->clock + parity;
+    p.clock_next = h->clock + (parity ^ 1);
     launch_step(h->Q, h->TX, h->TY, p, h->grid(), h->stream, h->cfg.storage, h->means(), false);
-    // the step counter feeds the time means and the FP16 stochastic rounding
-    if (h->means() || h->cfg.storage == LBM_STORE_FP16) {
-        hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, h->stream, h->clock);
-        HIPX(hipGetLastError());
-    }
 }
 
 hipGraphExec_t graph_for(lbm_handle *h) {
@@ -515,7 +514,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
             HIPX(hipMalloc(&h->faces[s], sizeof(float) * h->fwords));
             HIPX(hipMemsetAsync(h->faces[s], 0, sizeof(float) * h->fwords, h->stream));
         }
-        HIPX(hipMalloc(&h->clock, sizeof(StepClock)));
+        HIPX(hipMalloc(&h->clock, 2 * sizeof(StepClock)));
         set_clock(h);
         rebuild_faces(h);
         HIPX(hipStreamSynchronize(h->stream));

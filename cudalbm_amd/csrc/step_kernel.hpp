@@ -36,8 +36,11 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Step counter read by the fused time means: t of the step about to run and
-// invNewCount = 1/(t+1) (kineticEnergy.cuh:249), advanced by tick_kernel.
+// Step counter read by the fused time means and the FP16 stochastic rounding:
+// t of the step about to run and invNewCount = 1/(t+1) (kineticEnergy.cuh:249).
+// Two of them alternate with the face-set parity: a step reads its own and
+// one thread of the step writes the next step's (t+1), which no other
+// workgroup of the same launch reads -- no separate kernel per step.
 struct StepClock {
     int64_t t;
     double inv;
@@ -56,6 +59,7 @@ struct StepParams {
     double *kmean;
     double *smean;         // 6 component arrays, each of n doubles
     const StepClock *clock; // absolute step index of this step and 1/(t+1) (device)
+    StepClock *clock_next;  // the next step's clock, or null when nothing reads it
     float s_diag, s_off;    // strain-rate coefficients 1/(3 tau), 1/(6 tau)
     int64_t n;             // nodes
     // probe outputs
@@ -448,6 +452,13 @@ __device__ __forceinline__ void fo_values(const float m[10], float s_diag, float
 template <int Q, int TX, int TY, typename ST, bool FO, bool PROBE>
 __global__ __launch_bounds__(TX *TY, Q == 19 ? 4 : 3) void step_kernel(const StepParams p) {
     const int tid = threadIdx.x;
+    if constexpr (FO || sizeof(ST) == 2) {
+        if (blockIdx.x == 0 && tid == 0 && p.clock_next) {
+            const int64_t t = p.clock->t + 1;
+            p.clock_next->t = t;
+            p.clock_next->inv = 1.0 / double(t + 1);
+        }
+    }
 #define LBM_WORK_ITEM                      \
     const int b = blockIdx.x;              \
     const int bx = b % p.nbx;              \
