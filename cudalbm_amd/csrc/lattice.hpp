@@ -242,22 +242,26 @@ __device__ __forceinline__ void interior_moments(const Sums &s, float (&m)[10]) 
 template <int Q>
 __device__ __forceinline__ void boundary_moments(const Sums &s, int bits, float omega, float U,
                                                  float (&m)[10]) {
-    const float rI = s.s0 / d3<Q>();
-    const float axy = s.sxy / d3<Q>(), axz = s.sxz / d3<Q>(), ayz = s.syz / d3<Q>();
+    // divisions as reciprocal products (recip: hardware reciprocal + one Newton
+    // step, ~0.5 ulp; div_d3: the two-constant product): these lanes run the
+    // closure beside the interior lanes of their wave, so its cost is paid by
+    // every wave with a boundary node in it
+    const float rI = div_d3<Q>(s.s0);
+    const float axy = div_d3<Q>(s.sxy), axz = div_d3<Q>(s.sxz), ayz = div_d3<Q>(s.syz);
     const float w = omega, tw = 1.0f - omega;
     const bool north = (bits & 8) != 0;
     const int nb = __popc(bits & 0x3F);
-    const float ux = north ? U / float(nb) : 0.f;  // U<X> (boundaryConditions.cuh:543-554)
+    const float ux = north ? U * (nb == 1 ? 1.f : (nb == 2 ? 0.5f : 1.f / 3.f)) : 0.f;  // U<X> (boundaryConditions.cuh:543-554)
     float rp = 0.f, mxy = 0.f, mxz = 0.f, myz = 0.f;
     // static edge: sign +1 is rho = 36(rho_I - a + a w)/(24 + w), mab = (36a - rho)/(9 rho);
     // sign -1 is rho = -36(-rho_I - a + a w)/(24 + w), mab = (36a + rho)/(9 rho)
     auto edge = [&](float sign, float a, float &mab) {
-        rp = 36.f * (rI - sign * tw * a) / (24.f + w);
-        mab = (36.f * a - sign * rp) / (9.f * (1.f + rp));
+        rp = 36.f * (rI - sign * tw * a) * recip(24.f + w);
+        mab = (36.f * a - sign * rp) * recip(9.f * (1.f + rp));
     };
     switch (bits & 0x3F) {
     case 21: case 37: case 22: case 38:  // SWB SWF SEB SEF: rho = 12 rho_I/7 (216/125 for D3Q27)
-        rp = (Q == 19) ? 12.f * rI / 7.f : 216.f * rI / 125.f;
+        rp = rI * ((Q == 19) ? 12.f / 7.f : 216.f / 125.f);
         break;
     case 5: edge(+1.f, axy, mxy); break;   // SW
     case 6: edge(-1.f, axy, mxy); break;   // SE
@@ -268,62 +272,63 @@ __device__ __forceinline__ void boundary_moments(const Sums &s, int bits, float 
     case 20: edge(+1.f, ayz, myz); break;  // SB
     case 36: edge(-1.f, ayz, myz); break;  // SF
     case 1: case 2: {  // W, E: rho = 6 rho_I/5, mab = 2 a/rho
-        rp = 6.f * rI / 5.f;
-        const float rho = 1.f + rp;
-        mxy = 2.f * axy / rho;
-        mxz = 2.f * axz / rho;
+        rp = 1.2f * rI;
+        const float ir = recip(1.f + rp);
+        mxy = 2.f * axy * ir;
+        mxz = 2.f * axz * ir;
     } break;
     case 4: {  // S
-        rp = 6.f * rI / 5.f;
-        const float rho = 1.f + rp;
-        mxy = 2.f * axy / rho;
-        myz = 2.f * ayz / rho;
+        rp = 1.2f * rI;
+        const float ir = recip(1.f + rp);
+        mxy = 2.f * axy * ir;
+        myz = 2.f * ayz * ir;
     } break;
     case 16: case 32: {  // B, F
-        rp = 6.f * rI / 5.f;
-        const float rho = 1.f + rp;
-        mxz = 2.f * axz / rho;
-        myz = 2.f * ayz / rho;
+        rp = 1.2f * rI;
+        const float ir = recip(1.f + rp);
+        mxz = 2.f * axz * ir;
+        myz = 2.f * ayz * ir;
     } break;
     case 8: {  // N (moving lid): mxy = (6a - U rho)/(3 rho), myz = 2a/rho
-        rp = 6.f * rI / 5.f;
+        rp = 1.2f * rI;
         const float rho = 1.f + rp;
-        mxy = (6.f * axy - U * rho) / (3.f * rho);
-        myz = 2.f * ayz / rho;
+        const float ir = recip(rho);
+        mxy = (6.f * axy - U * rho) * (ir * (1.f / 3.f));
+        myz = 2.f * ayz * ir;
     } break;
     case 25: case 41: {  // NWB NWF: rho = 24 rho_I/(14 + 8U - 9U^2); W = 7/12 (125/216 for D3Q27)
         const float c0 = (Q == 19) ? 0.f : -1.f / 9.f;
-        rp = (24.f * rI + c0 - 8.f * U + 9.f * U * U) / (14.f + 8.f * U - 9.f * U * U);
+        rp = (24.f * rI + c0 - 8.f * U + 9.f * U * U) * recip(14.f + 8.f * U - 9.f * U * U);
     } break;
     case 26: case 42: {  // NEB NEF: rho = 24 rho_I/(14 - 8U - 9U^2)
         const float c0 = (Q == 19) ? 0.f : -1.f / 9.f;
-        rp = (24.f * rI + c0 + 8.f * U + 9.f * U * U) / (14.f - 8.f * U - 9.f * U * U);
+        rp = (24.f * rI + c0 + 8.f * U + 9.f * U * U) * recip(14.f - 8.f * U - 9.f * U * U);
     } break;
     case 24: {  // NB: rho = 72(-rho_I - a + a w)/(-48 - 2w + 3U^2 w)
         const float g = -48.f - 2.f * w + 3.f * U * U * w;
-        rp = (-72.f * rI - 72.f * tw * ayz - 3.f * U * U * w) / g;
+        rp = (-72.f * rI - 72.f * tw * ayz - 3.f * U * U * w) * recip(g);
         const float rho = 1.f + rp;
-        myz = (72.f * ayz + 2.f * rp - 3.f * U * U * rho) / (18.f * rho);
+        myz = (72.f * ayz + 2.f * rp - 3.f * U * U * rho) * recip(18.f * rho);
     } break;
     case 40: {  // NF: rho = -72(rho_I - a + a w)/(-48 - 2w + 3U^2 w)
         const float g = -48.f - 2.f * w + 3.f * U * U * w;
-        rp = (-72.f * rI + 72.f * tw * ayz - 3.f * U * U * w) / g;
+        rp = (-72.f * rI + 72.f * tw * ayz - 3.f * U * U * w) * recip(g);
         const float rho = 1.f + rp;
-        myz = (72.f * ayz - 2.f * rp + 3.f * U * U * rho) / (18.f * rho);
+        myz = (72.f * ayz - 2.f * rp + 3.f * U * U * rho) * recip(18.f * rho);
     } break;
     case 10: {  // NE: rho = 36(rho_I - a + a w)/g
         const float g = 24.f - 18.f * U - 18.f * U * U + w + 3.f * U * w + 3.f * U * U * w;
         const float c0 = 18.f * U + 18.f * U * U - 3.f * U * w - 3.f * U * U * w;  // 24 + w - g
-        rp = (36.f * rI - 36.f * tw * axy + c0) / g;
+        rp = (36.f * rI - 36.f * tw * axy + c0) * recip(g);
         const float rho = 1.f + rp;
-        mxy = (36.f * axy - rp - 3.f * U * rho - 3.f * U * U * rho) / (9.f * rho);
+        mxy = (36.f * axy - rp - 3.f * U * rho - 3.f * U * U * rho) * recip(9.f * rho);
     } break;
     case 9: {  // NW: rho = -36(-rho_I - a + a w)/g
         const float g = 24.f + 18.f * U - 18.f * U * U + w - 3.f * U * w + 3.f * U * U * w;
         const float c0 = -18.f * U + 18.f * U * U + 3.f * U * w - 3.f * U * U * w;  // 24 + w - g
-        rp = (36.f * rI + 36.f * tw * axy + c0) / g;
+        rp = (36.f * rI + 36.f * tw * axy + c0) * recip(g);
         const float rho = 1.f + rp;
-        mxy = (36.f * axy + rp - 3.f * U * rho + 3.f * U * U * rho) / (9.f * rho);
+        mxy = (36.f * axy + rp - 3.f * U * rho + 3.f * U * U * rho) * recip(9.f * rho);
     } break;
     default: break;
     }
