@@ -238,110 +238,142 @@ __device__ __forceinline__ void interior_moments(const Sums &s, float (&m)[10]) 
 // rho - 1 = (alpha rI + beta a + c0)/gamma with c0 = alpha W + beta A - gamma,
 // which is identically zero for every static wall, edge and corner (their
 // closures keep the rest state), so it is dropped there and kept in closed form
-// for the lid.  `bits` is the normalVector bitmask (W=1, E=2, S=4, N=8, B=16, F=32).
+// for the lid.  `code` is the normalVector bitmask without its boundary bit
+// (W=1, E=2, S=4, N=8, B=16, F=32).
+//
+// All 26 closures share one form, so they are evaluated from a per-case
+// coefficient row instead of a switch (a wave holding lanes of several cases
+// would otherwise run each case's branch in turn, next to its interior lanes):
+//   rho - 1 = (A rI + Bxy axy + Bxz axz + Byz ayz + C) / D
+//   m_k     = (E_k a_k + F_k (rho - 1) + G_k rho) / (H_k rho),  k = xy, xz, yz
+// with the stored velocity u_x = U / (number of boundaries) on lid nodes
+// (boundaryConditions.cuh:543-554) and m_xx = u_x^2, m_yy = m_zz = 0.
+struct Closure {
+    float A, Bxy, Bxz, Byz, C, invD, ux, pad;
+    float E[3], F[3], G[3], invH[3];
+};
+
+// Row of a node code in a 27-row closure table: one ternary digit per axis
+// (0: inside, 1: W/S/B wall, 2: E/N/F wall).
+__host__ __device__ __forceinline__ int closure_row(int code) {
+    const int ix = (code & 1) ? 1 : ((code & 2) ? 2 : 0);
+    const int iy = (code & 4) ? 1 : ((code & 8) ? 2 : 0);
+    const int iz = (code & 16) ? 1 : ((code & 32) ? 2 : 0);
+    return ix + 3 * iy + 9 * iz;
+}
+__host__ __device__ __forceinline__ int closure_code(int row) {
+    const int ix = row % 3, iy = (row / 3) % 3, iz = row / 9;
+    return (ix == 1 ? 1 : ix == 2 ? 2 : 0) | (iy == 1 ? 4 : iy == 2 ? 8 : 0) | (iz == 1 ? 16 : iz == 2 ? 32 : 0);
+}
+
 template <int Q>
-__device__ __forceinline__ void boundary_moments(const Sums &s, int bits, float omega, float U,
-                                                 float (&m)[10]) {
-    // divisions as reciprocal products (recip: hardware reciprocal + one Newton
-    // step, ~0.5 ulp; div_d3: the two-constant product): these lanes run the
-    // closure beside the interior lanes of their wave, so its cost is paid by
-    // every wave with a boundary node in it
-    const float rI = div_d3<Q>(s.s0);
-    const float axy = div_d3<Q>(s.sxy), axz = div_d3<Q>(s.sxz), ayz = div_d3<Q>(s.syz);
-    const float w = omega, tw = 1.0f - omega;
-    const bool north = (bits & 8) != 0;
-    const int nb = __popc(bits & 0x3F);
-    const float ux = north ? U * (nb == 1 ? 1.f : (nb == 2 ? 0.5f : 1.f / 3.f)) : 0.f;  // U<X> (boundaryConditions.cuh:543-554)
-    float rp = 0.f, mxy = 0.f, mxz = 0.f, myz = 0.f;
-    // static edge: sign +1 is rho = 36(rho_I - a + a w)/(24 + w), mab = (36a - rho)/(9 rho);
-    // sign -1 is rho = -36(-rho_I - a + a w)/(24 + w), mab = (36a + rho)/(9 rho)
-    auto edge = [&](float sign, float a, float &mab) {
-        rp = 36.f * (rI - sign * tw * a) * recip(24.f + w);
-        mab = (36.f * a - sign * rp) * recip(9.f * (1.f + rp));
+__device__ inline Closure closure_coeffs(int code, float U, float w) {
+    Closure k{};
+    k.invH[0] = k.invH[1] = k.invH[2] = 1.f;
+    k.invD = 1.f;
+    const float tw = 1.0f - w, U2 = U * U;
+    const int nb = __popc(code);
+    k.ux = (code & 8) ? U / float(nb) : 0.f;
+    // static edge: rho = 36(rho_I - s tw a)/(24 + w), m = (36 a - s rho')/(9 rho)
+    auto edge = [&](float sign, int a) {
+        k.A = 36.f;
+        (a == 0 ? k.Bxy : a == 1 ? k.Bxz : k.Byz) = -36.f * sign * tw;
+        k.invD = 1.f / (24.f + w);
+        k.E[a] = 36.f;
+        k.F[a] = -sign;
+        k.invH[a] = 1.f / 9.f;
     };
-    switch (bits & 0x3F) {
+    switch (code) {
     case 21: case 37: case 22: case 38:  // SWB SWF SEB SEF: rho = 12 rho_I/7 (216/125 for D3Q27)
-        rp = rI * ((Q == 19) ? 12.f / 7.f : 216.f / 125.f);
+        k.A = (Q == 19) ? 12.f / 7.f : 216.f / 125.f;
         break;
-    case 5: edge(+1.f, axy, mxy); break;   // SW
-    case 6: edge(-1.f, axy, mxy); break;   // SE
-    case 17: edge(+1.f, axz, mxz); break;  // WB
-    case 33: edge(-1.f, axz, mxz); break;  // WF
-    case 18: edge(-1.f, axz, mxz); break;  // EB
-    case 34: edge(+1.f, axz, mxz); break;  // EF
-    case 20: edge(+1.f, ayz, myz); break;  // SB
-    case 36: edge(-1.f, ayz, myz); break;  // SF
-    case 1: case 2: {  // W, E: rho = 6 rho_I/5, mab = 2 a/rho
-        rp = 1.2f * rI;
-        const float ir = recip(1.f + rp);
-        mxy = 2.f * axy * ir;
-        mxz = 2.f * axz * ir;
-    } break;
-    case 4: {  // S
-        rp = 1.2f * rI;
-        const float ir = recip(1.f + rp);
-        mxy = 2.f * axy * ir;
-        myz = 2.f * ayz * ir;
-    } break;
-    case 16: case 32: {  // B, F
-        rp = 1.2f * rI;
-        const float ir = recip(1.f + rp);
-        mxz = 2.f * axz * ir;
-        myz = 2.f * ayz * ir;
-    } break;
-    case 8: {  // N (moving lid): mxy = (6a - U rho)/(3 rho), myz = 2a/rho
-        rp = 1.2f * rI;
-        const float rho = 1.f + rp;
-        const float ir = recip(rho);
-        mxy = (6.f * axy - U * rho) * (ir * (1.f / 3.f));
-        myz = 2.f * ayz * ir;
-    } break;
+    case 5: edge(+1.f, 0); break;   // SW
+    case 6: edge(-1.f, 0); break;   // SE
+    case 17: edge(+1.f, 1); break;  // WB
+    case 33: edge(-1.f, 1); break;  // WF
+    case 18: edge(-1.f, 1); break;  // EB
+    case 34: edge(+1.f, 1); break;  // EF
+    case 20: edge(+1.f, 2); break;  // SB
+    case 36: edge(-1.f, 2); break;  // SF
+    case 1: case 2:  // W, E: rho = 6 rho_I/5, mab = 2 a/rho
+        k.A = 1.2f;
+        k.E[0] = k.E[1] = 2.f;
+        break;
+    case 4:  // S
+        k.A = 1.2f;
+        k.E[0] = k.E[2] = 2.f;
+        break;
+    case 16: case 32:  // B, F
+        k.A = 1.2f;
+        k.E[1] = k.E[2] = 2.f;
+        break;
+    case 8:  // N (moving lid): mxy = (6a - U rho)/(3 rho), myz = 2a/rho
+        k.A = 1.2f;
+        k.E[0] = 6.f;
+        k.G[0] = -U;
+        k.invH[0] = 1.f / 3.f;
+        k.E[2] = 2.f;
+        break;
     case 25: case 41: {  // NWB NWF: rho = 24 rho_I/(14 + 8U - 9U^2); W = 7/12 (125/216 for D3Q27)
-        const float c0 = (Q == 19) ? 0.f : -1.f / 9.f;
-        rp = (24.f * rI + c0 - 8.f * U + 9.f * U * U) * recip(14.f + 8.f * U - 9.f * U * U);
+        k.A = 24.f;
+        k.C = ((Q == 19) ? 0.f : -1.f / 9.f) - 8.f * U + 9.f * U2;
+        k.invD = 1.f / (14.f + 8.f * U - 9.f * U2);
     } break;
     case 26: case 42: {  // NEB NEF: rho = 24 rho_I/(14 - 8U - 9U^2)
-        const float c0 = (Q == 19) ? 0.f : -1.f / 9.f;
-        rp = (24.f * rI + c0 + 8.f * U + 9.f * U * U) * recip(14.f - 8.f * U - 9.f * U * U);
+        k.A = 24.f;
+        k.C = ((Q == 19) ? 0.f : -1.f / 9.f) + 8.f * U + 9.f * U2;
+        k.invD = 1.f / (14.f - 8.f * U - 9.f * U2);
     } break;
-    case 24: {  // NB: rho = 72(-rho_I - a + a w)/(-48 - 2w + 3U^2 w)
-        const float g = -48.f - 2.f * w + 3.f * U * U * w;
-        rp = (-72.f * rI - 72.f * tw * ayz - 3.f * U * U * w) * recip(g);
-        const float rho = 1.f + rp;
-        myz = (72.f * ayz + 2.f * rp - 3.f * U * U * rho) * recip(18.f * rho);
+    case 24: case 40: {  // NB, NF: rho = -/+72(rho_I -/+ a + a w)/(-48 - 2w + 3U^2 w)
+        const float s = code == 24 ? 1.f : -1.f;
+        k.A = -72.f;
+        k.Byz = -72.f * s * tw;
+        k.C = -3.f * U2 * w;
+        k.invD = 1.f / (-48.f - 2.f * w + 3.f * U2 * w);
+        k.E[2] = 72.f;
+        k.F[2] = 2.f * s;
+        k.G[2] = -3.f * U2 * s;
+        k.invH[2] = 1.f / 18.f;
     } break;
-    case 40: {  // NF: rho = -72(rho_I - a + a w)/(-48 - 2w + 3U^2 w)
-        const float g = -48.f - 2.f * w + 3.f * U * U * w;
-        rp = (-72.f * rI + 72.f * tw * ayz - 3.f * U * U * w) * recip(g);
-        const float rho = 1.f + rp;
-        myz = (72.f * ayz - 2.f * rp + 3.f * U * U * rho) * recip(18.f * rho);
-    } break;
-    case 10: {  // NE: rho = 36(rho_I - a + a w)/g
-        const float g = 24.f - 18.f * U - 18.f * U * U + w + 3.f * U * w + 3.f * U * U * w;
-        const float c0 = 18.f * U + 18.f * U * U - 3.f * U * w - 3.f * U * U * w;  // 24 + w - g
-        rp = (36.f * rI - 36.f * tw * axy + c0) * recip(g);
-        const float rho = 1.f + rp;
-        mxy = (36.f * axy - rp - 3.f * U * rho - 3.f * U * U * rho) * recip(9.f * rho);
-    } break;
-    case 9: {  // NW: rho = -36(-rho_I - a + a w)/g
-        const float g = 24.f + 18.f * U - 18.f * U * U + w - 3.f * U * w + 3.f * U * U * w;
-        const float c0 = -18.f * U + 18.f * U * U + 3.f * U * w - 3.f * U * U * w;  // 24 + w - g
-        rp = (36.f * rI + 36.f * tw * axy + c0) * recip(g);
-        const float rho = 1.f + rp;
-        mxy = (36.f * axy + rp - 3.f * U * rho + 3.f * U * U * rho) * recip(9.f * rho);
+    case 10: case 9: {  // NE, NW: rho = 36(rho_I -/+ a +/- a w)/g
+        const float s = code == 10 ? 1.f : -1.f;  // NE: +1
+        const float g = 24.f - 18.f * s * U - 18.f * U2 + w + 3.f * s * U * w + 3.f * U2 * w;
+        k.A = 36.f;
+        k.Bxy = -36.f * s * tw;
+        k.C = 18.f * s * U + 18.f * U2 - 3.f * s * U * w - 3.f * U2 * w;  // 24 + w - g
+        k.invD = 1.f / g;
+        k.E[0] = 36.f;
+        k.F[0] = -s;
+        k.G[0] = -3.f * U - 3.f * s * U2;
+        k.invH[0] = 1.f / 9.f;
     } break;
     default: break;
     }
-    // unscaled -> scaled (velocitySet.cuh:130-142); m_xx = u_x^2, m_yy = m_zz = 0
+    return k;
+}
+
+// Moments of a boundary node from its known sums and its closure row
+// (reciprocals: hardware reciprocal + one Newton step; 1/(3D) as a two-term
+// product), scaled as stored (velocitySet.cuh:130-142).
+template <int Q>
+__device__ __forceinline__ void boundary_moments(const Sums &s, const Closure &k, float (&m)[10]) {
+    const float rI = div_d3<Q>(s.s0);
+    const float a[3] = {div_d3<Q>(s.sxy), div_d3<Q>(s.sxz), div_d3<Q>(s.syz)};
+    const float rp = (k.A * rI + k.Bxy * a[0] + k.Bxz * a[1] + k.Byz * a[2] + k.C) * k.invD;
+    const float rho = 1.f + rp;
+    const float ir = recip(rho);
+    float mab[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) mab[j] = (k.E[j] * a[j] + k.F[j] * rp + k.G[j] * rho) * (k.invH[j] * ir);
     m[0] = rp;
-    m[1] = 3.f * ux;
+    m[1] = 3.f * k.ux;
     m[2] = 0.f;
     m[3] = 0.f;
-    m[4] = 4.5f * ux * ux;
-    m[5] = 9.f * mxy;
-    m[6] = 9.f * mxz;
+    m[4] = 4.5f * k.ux * k.ux;
+    m[5] = 9.f * mab[0];
+    m[6] = 9.f * mab[1];
     m[7] = 0.f;
-    m[8] = 9.f * myz;
+    m[8] = 9.f * mab[2];
     m[9] = 0.f;
 }
 
