@@ -79,12 +79,16 @@ __device__ __forceinline__ const void *byte_at(const void *p, unsigned off) {
     return static_cast<const char *>(p) + off;
 }
 __device__ __forceinline__ void *byte_at(void *p, unsigned off) { return static_cast<char *>(p) + off; }
+// Moment arrays are streamed once per step with the nontemporal cache policy,
+// which leaves the caches to the face buffers that the next step reads back
+// (face stores with the same policy made the FP32 D3Q19 step 8% slower).
 template <>
 struct Store<float> {
     static constexpr unsigned kBytes = 4;
     __device__ static __forceinline__ float load(const void *p, unsigned off, float) {
         return __builtin_nontemporal_load(static_cast<const float *>(byte_at(p, off)));
     }
+    __device__ static __forceinline__ float load_raw(const void *p, unsigned off) { return load(p, off, 1.f); }
     __device__ static __forceinline__ void store(void *p, unsigned off, float v) {
         __builtin_nontemporal_store(v, static_cast<float *>(byte_at(p, off)));
     }
@@ -93,10 +97,14 @@ template <>
 struct Store<_Float16> {
     static constexpr unsigned kBytes = 2;
     __device__ static __forceinline__ float load(const void *p, unsigned off, float dec) {
-        return float(*static_cast<const _Float16 *>(byte_at(p, off))) * dec;
+        return float(__builtin_nontemporal_load(static_cast<const _Float16 *>(byte_at(p, off)))) * dec;
+    }
+    // the stored value only: the step folds the decode scales into rec_setup_st
+    __device__ static __forceinline__ float load_raw(const void *p, unsigned off) {
+        return float(__builtin_nontemporal_load(static_cast<const _Float16 *>(byte_at(p, off))));
     }
     __device__ static __forceinline__ void store(void *p, unsigned off, _Float16 v) {
-        *static_cast<_Float16 *>(byte_at(p, off)) = v;
+        __builtin_nontemporal_store(v, static_cast<_Float16 *>(byte_at(p, off)));
     }
 };
 
@@ -347,22 +355,23 @@ struct Edge {
 };
 
 // x/y face populations of one finished plane, written cooperatively: edge
-// nodes park their post-collision moments in LDS (edge_m, 10 x NE words) and
+// nodes park their post-collision moments in LDS (edge_m, 10 x NE values as
+// stored: FP32, or the FP16 codes that rec_setup_st decodes) and
 // the first NE threads -- whole waves where TX is a multiple of 32 -- rebuild
 // and store the face populations, instead of every wave rebuilding them for
 // its few edge lanes.  Row entries write the y faces, column entries the x
 // faces, so a wave of row entries runs only the y-face branches.
-template <int Q, int TX, int TY, bool PROBE>
+template <int Q, int TX, int TY, bool PROBE, typename ST>
 __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__restrict__ fout,
-                                               const float *edge_m, int tid, int bx, int by, int z) {
+                                               const ST *edge_m, int tid, int bx, int by, int z) {
     using E = Edge<TX, TY>;
     if (tid >= E::NE) return;
     int tx, ty;
     E::coords(tid, tx, ty);
     float m[10];
 #pragma unroll
-    for (int k = 0; k < 10; k++) m[k] = edge_m[k * E::NE + tid];
-    const Rec r = rec_setup(m);
+    for (int k = 0; k < 10; k++) m[k] = float(edge_m[k * E::NE + tid]);
+    const Rec r = rec_setup_st<ST>(m, p);
     const int x = bx * TX + tx, y = by * TY + ty;
     if (tid < 2 * TX)
         write_y_faces<Q, TX, TY, PROBE>(p, fout, r, ty, x, y, z, by);
@@ -384,15 +393,54 @@ __device__ __forceinline__ void load_plane(const StepParams &p, int bx, int by, 
     for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], off, p.dec[k]);
 }
 
-// Next plane's old moments, or past the chunk's last plane the down-going
-// populations of the plane above it (bottom face of chunk c+1; zero above the
-// domain) in m[0..QF-1].
+// The same plane as stored (FP16: not yet scaled by the codec's decode
+// factors, which rec_setup_st applies).
+template <typename ST, int TX, int TY>
+__device__ __forceinline__ void load_plane_raw(const StepParams &p, int bx, int by, int z, int tid,
+                                               float (&m)[10]) {
+    const unsigned off = (plane_base<TX, TY>(p, bx, by, z) + (unsigned)tid) * Store<ST>::kBytes;
+#pragma unroll
+    for (int k = 0; k < 10; k++) m[k] = Store<ST>::load_raw(p.mom[k], off);
+}
+
+// Population-rebuild terms from a plane loaded by load_plane_raw: FP32 as
+// rec_setup; FP16 with the decode factors folded into the products (they are
+// equal within the groups {1,2,3}, {4,7,9}, {5,6,8}: the codec's u, Pi_aa and
+// Pi_ab scales).
+template <typename ST>
+__device__ __forceinline__ Rec rec_setup_st(const float (&h)[10], const StepParams &p) {
+    if constexpr (sizeof(ST) == 4) {
+        return rec_setup(h);
+    } else {
+        Rec r;
+        const float m0 = h[0] * p.dec[0];
+        const float rho = 1.0f + m0;
+        r.a = fmaf(rho, -(h[4] + h[7] + h[9]) * p.dec[4], 3.0f * m0);
+        const float t = 3.0f * rho;
+        const float tu = t * p.dec[1], td = t * p.dec[4], to = t * p.dec[5];
+        r.M[1] = tu * h[1];
+        r.M[2] = tu * h[2];
+        r.M[3] = tu * h[3];
+        r.M[4] = td * h[4];
+        r.M[7] = td * h[7];
+        r.M[9] = td * h[9];
+        r.M[5] = to * h[5];
+        r.M[6] = to * h[6];
+        r.M[8] = to * h[8];
+        r.M[0] = 0.f;
+        return r;
+    }
+}
+
+// Next plane's old moments as stored (load_plane_raw), or past the chunk's last
+// plane the down-going populations of the plane above it (bottom face of chunk
+// c+1; zero above the domain) in m[0..QF-1].
 template <int Q, typename ST, int TX, int TY, bool PROBE>
 __device__ __forceinline__ void load_next(const StepParams &p, const float *__restrict__ fin, int bx, int by,
                                           int c, int zn, int z1, int tid, int x, int y, float (&m)[10]) {
     if (zn <= z1) {
         if constexpr (!PROBE) {
-            load_plane<ST, TX, TY>(p, bx, by, zn, tid, m);
+            load_plane_raw<ST, TX, TY>(p, bx, by, zn, tid, m);
         } else {
 #pragma unroll
             for (int k = 0; k < 10; k++) m[k] = 0.f;
