@@ -109,25 +109,29 @@ int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_
 // Planes per work item.  A launch runs in rounds of `slots` resident
 // workgroups; a work item of lz planes costs about (lz + 4) plane times (the
 // 4: pipeline fill and drain, ring and z-face set-up) at bpn(lz) bytes per
-// node, bpn = 80 + 16 QF (1/TX + 1/TY + 1/lz) with the face words.  The
-// choice minimises rounds x (lz + 4) x bpn(lz) over the divisors of nz in
-// [4, 64], with a 10% penalty for a single round (no other workgroup overlaps
-// the first and last planes).  Measured on MI355X (MLUPS, best choice = the
-// model's): D3Q19 32x16 at 128^3 lz 8 (45.0k; lz 4: 40.9k, 16: 40.5k), 192^3
-// lz 16 (52.8k; 24: 43.7k), 256^3 lz 32 (63.0k; 64: 61.6k); D3Q27 16x16 at
-// 128^3 lz 16 (31.4k; 8: 27.7k), 192^3 lz 12 (40.0k; 16: 37.8k), 256^3 lz 32
-// (42.6k; 64: 38.4k), 384^3 lz 48 (45.1k; 64: 44.0k).
-int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots) {
+// node, bpn = M + 16 QF (1/TX + 1/TY + 1/lz) with M = 80 moment bytes (40 in
+// FP16) and the face words.  The choice minimises rounds x (lz + 4) x bpn(lz)
+// over the divisors of nz in [4, 64]; with FP32 storage a single round costs
+// 10% more (no other workgroup overlaps the first and last planes of the
+// HBM-bound step), with FP16 it does not (its step is instruction-bound).
+// Measured on MI355X (MLUPS, best choice = the model's): D3Q19 32x16 at 128^3
+// lz 8 (45.0k; lz 4: 40.9k, 16: 40.5k), 192^3 lz 16 (52.8k; 24: 43.7k), 256^3
+// lz 32 (63.0k; 64: 61.6k), FP16 256^3 lz 64 (85.4k; 32: 81.9k); D3Q27 16x16
+// at 128^3 lz 16 (31.4k; 8: 27.7k), 192^3 lz 12 (40.0k; 16: 37.8k), 256^3 lz
+// 32 (42.6k; 64: 38.4k), 384^3 lz 48 (45.1k; 64: 44.0k); D3Q27 FP16 32x8 at
+// 256^3 lz 32 (49.1k; 16: 46.6k, 64: 42.1k), 384^3 lz 48 (53.7k; 32: 52.7k).
+int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool fp16) {
     const int64_t cols = (int64_t)(nx / tx) * (ny / ty);
     const double face = 16.0 * (Q == 19 ? 5 : 9);
+    const double mom = fp16 ? 40.0 : 80.0;
     int best = -1;
     double best_cost = 0.0;
     for (int lz = 4; lz <= std::min(nz, 64); lz++) {
         if (nz % lz) continue;
         const int64_t items = cols * (nz / lz);
         const int64_t rounds = (items + slots - 1) / slots;
-        const double bpn = 80.0 + face * (1.0 / tx + 1.0 / ty + 1.0 / lz);
-        const double cost = (double)rounds * (lz + 4) * bpn * (rounds == 1 ? 1.1 : 1.0);
+        const double bpn = mom + face * (1.0 / tx + 1.0 / ty + 1.0 / lz);
+        const double cost = (double)rounds * (lz + 4) * bpn * (rounds == 1 && !fp16 ? 1.1 : 1.0);
         if (best < 0 || cost < best_cost) {
             best = lz;
             best_cost = cost;
@@ -150,10 +154,14 @@ int64_t resident_workgroups(int Q, int tx, int ty, int device) {
     return (int64_t)cus * per_cu;
 }
 
-void choose_tile(int Q, int nx, int ny, int *tx, int *ty) {
+void choose_tile(int Q, int nx, int ny, bool fp16, int *tx, int *ty) {
     // measured on MI355X at 256^3: D3Q19 is fastest with 32x16 footprints (8 waves,
-    // fewest face words per node); D3Q27's LDS allows 16x16 at 3 workgroups/CU
+    // fewest face words per node); D3Q27's LDS allows 16x16 at 3 workgroups/CU.
+    // D3Q27 with FP16 storage runs 9% faster at 32x8 (256^3: 342 vs 373 us;
+    // 384^3: 1054 vs 1152 us); with FP32 storage 16x16 stays ahead at 384^3
+    // (1234 vs 1294 us; 256^3: 396 vs 390 us).
     if (Q == 19 && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
+    else if (Q == 27 && fp16 && nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
     else if (nx % 16 == 0 && ny % 16 == 0) { *tx = 16; *ty = 16; }
     else if (nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
     else { *tx = 8; *ty = 8; }
@@ -476,7 +484,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     if (!(c.Re > 0) || !std::isfinite(c.u_inf)) return bad(LBM_ERR_ARG, "Re must be > 0 and u_inf finite");
     h->Q = c.velocity_set;
     if (c.tile_x == 0 || c.tile_y == 0)
-        choose_tile(h->Q, c.nx, c.ny, &h->TX, &h->TY);
+        choose_tile(h->Q, c.nx, c.ny, c.storage == LBM_STORE_FP16, &h->TX, &h->TY);
     else {
         h->TX = c.tile_x;
         h->TY = c.tile_y;
@@ -490,7 +498,8 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     if (h->n > ((int64_t)1 << 30)) return bad(LBM_ERR_SIZE, "mesh too large: at most 2^30 nodes (4 GiB per moment array)");
     h->LZ = c.tile_z > 0 ? c.tile_z
                          : choose_lz(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY,
-                                     resident_workgroups(h->Q, h->TX, h->TY, c.device));
+                                     resident_workgroups(h->Q, h->TX, h->TY, c.device),
+                                     c.storage == LBM_STORE_FP16);
     if (c.nz % h->LZ) return bad(LBM_ERR_SIZE, "nz must be a multiple of tile_z");
     h->nbx = c.nx / h->TX;
     h->nby = c.ny / h->TY;
