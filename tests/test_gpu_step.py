@@ -168,3 +168,32 @@ def test_parity_d3q27_128_re1000_fp32_fp16(native_lib):
         print(f"Q27 128^3 Re 1000 {steps} steps {storage}: velocity {ev:.3e} rho {er:.3e}", flush=True)
         assert ev <= bound
         assert er <= bound_rho
+
+
+@pytest.mark.parametrize("Q,storage,expected", [
+    (19, "fp32", (32, 16, 32)),
+    (19, "fp16", (32, 16, 64)),
+    (27, "fp32", (16, 16, 32)),
+    (27, "fp16", (32, 8, 32)),
+])
+def test_automatic_tiling_256(native_lib, Q, storage, expected):
+    """The automatic work decomposition at 256^3: the footprint and planes per
+    work item that measured fastest for each velocity set and storage (FP16:
+    no single-round penalty, D3Q27 on 32x8).  The footprint does not depend
+    on the device; the planes per item do (resident workgroups), so they are
+    checked on a 256-CU device only."""
+    import ctypes
+
+    from cudalbm_amd import Solver
+
+    # the HIP runtime the library already loaded; 63 = hipDeviceAttributeMultiprocessorCount
+    hip = ctypes.CDLL("libamdhip64.so")
+    cus = ctypes.c_int(0)
+    assert hip.hipDeviceGetAttribute(ctypes.byref(cus), 63, 0) == 0
+    cus = cus.value
+    with Solver(256, 256, 256, 1000.0, 0.05, velocity_set=Q, storage=storage) as s:
+        got = s.tiling
+    print(f"D3Q{Q} {storage} 256^3 on {cus} CUs: tiling {got}", flush=True)
+    assert got[:2] == expected[:2]
+    if cus == 256:
+        assert got[2] == expected[2]
