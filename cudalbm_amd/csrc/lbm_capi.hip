@@ -261,13 +261,11 @@ struct lbm_handle {
         return p;
     }
     void codec(float *enc, float *dec) const {
-        // FP16 storage: (rho-1)/cs2, u/cs, (Pi_ab/rho - cs2 d_ab)/cs2
-        static const double e[10] = {3.0, 1.0 / std::sqrt(3.0), 1.0 / std::sqrt(3.0), 1.0 / std::sqrt(3.0),
-                                     2.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0};
+        // FP16 storage: the step's compile-time codec (step_kernel.hpp)
+        const bool h16 = cfg.storage == LBM_STORE_FP16;
         for (int i = 0; i < 10; i++) {
-            const double v = cfg.storage == LBM_STORE_FP16 ? e[i] : 1.0;
-            enc[i] = (float)v;
-            dec[i] = (float)(1.0 / v);
+            enc[i] = h16 ? codec_enc(i) : 1.f;
+            dec[i] = h16 ? codec_dec(i) : 1.f;
         }
     }
     int grid() const { return nbx * nby * nc; }

@@ -53,7 +53,7 @@ struct StepParams {
     const float *fin;      // face buffers read (previous step): X | Y | Z
     float *fout;           // face buffers written (this step)
     int yoff, zoff;        // offsets of the Y and Z face blocks inside one face set
-    float enc[10], dec[10];  // FP16 storage codec: stored16 = m*enc, m = stored16*dec
+    float enc[10], dec[10];  // codec_enc/codec_dec (FP16) or 1 (FP32), for the auxiliary kernels
     // fused FP64 time means of the function objects (fo_mask bits 2: kMean, 8: SMean)
     unsigned fo_mask;
     double *kmean;
@@ -66,6 +66,15 @@ struct StepParams {
     uint32_t *probe_src;
     uint8_t *probe_type;
 };
+
+// FP16 storage codec: stored16 = m * codec_enc(k), m = stored16 * codec_dec(k);
+// (rho-1)/cs2, u/cs, (Pi_ab/rho - cs2 d_ab)/cs2 with the stored 4.5/9 factors
+// folded in.  Compile-time constants, so the step holds no codec registers.
+__host__ __device__ constexpr double codec_enc_d(int k) {
+    return k == 0 ? 3.0 : k <= 3 ? 0.57735026918962576451 : (k == 4 || k == 7 || k == 9) ? 2.0 / 3.0 : 1.0 / 3.0;
+}
+__host__ __device__ constexpr float codec_enc(int k) { return (float)codec_enc_d(k); }
+__host__ __device__ constexpr float codec_dec(int k) { return (float)(1.0 / codec_enc_d(k)); }
 
 template <typename ST>
 struct Store;
@@ -390,7 +399,7 @@ __device__ __forceinline__ void load_plane(const StepParams &p, int bx, int by, 
                                            float (&m)[10]) {
     const unsigned off = (plane_base<TX, TY>(p, bx, by, z) + (unsigned)tid) * Store<ST>::kBytes;
 #pragma unroll
-    for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], off, p.dec[k]);
+    for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], off, codec_dec(k));
 }
 
 // The same plane as stored (FP16: not yet scaled by the codec's decode
@@ -413,11 +422,11 @@ __device__ __forceinline__ Rec rec_setup_st(const float (&h)[10], const StepPara
         return rec_setup(h);
     } else {
         Rec r;
-        const float m0 = h[0] * p.dec[0];
+        const float m0 = h[0] * codec_dec(0);
         const float rho = 1.0f + m0;
-        r.a = fmaf(rho, -(h[4] + h[7] + h[9]) * p.dec[4], 3.0f * m0);
+        r.a = fmaf(rho, -(h[4] + h[7] + h[9]) * codec_dec(4), 3.0f * m0);
         const float t = 3.0f * rho;
-        const float tu = t * p.dec[1], td = t * p.dec[4], to = t * p.dec[5];
+        const float tu = t * codec_dec(1), td = t * codec_dec(4), to = t * codec_dec(5);
         r.M[1] = tu * h[1];
         r.M[2] = tu * h[2];
         r.M[3] = tu * h[3];
