@@ -363,6 +363,36 @@ struct Edge {
     }
 };
 
+// Population-rebuild terms from moments as stored (load_plane_raw, the edge
+// codes of flush_xy_faces): FP32 as
+// rec_setup; FP16 with the decode factors folded into the products (they are
+// equal within the groups {1,2,3}, {4,7,9}, {5,6,8}: the codec's u, Pi_aa and
+// Pi_ab scales).
+template <typename ST>
+__device__ __forceinline__ Rec rec_setup_st(const float (&h)[10]) {
+    if constexpr (sizeof(ST) == 4) {
+        return rec_setup(h);
+    } else {
+        Rec r;
+        const float m0 = h[0] * codec_dec(0);
+        const float rho = 1.0f + m0;
+        r.a = fmaf(rho, -(h[4] + h[7] + h[9]) * codec_dec(4), 3.0f * m0);
+        const float t = 3.0f * rho;
+        const float tu = t * codec_dec(1), td = t * codec_dec(4), to = t * codec_dec(5);
+        r.M[1] = tu * h[1];
+        r.M[2] = tu * h[2];
+        r.M[3] = tu * h[3];
+        r.M[4] = td * h[4];
+        r.M[7] = td * h[7];
+        r.M[9] = td * h[9];
+        r.M[5] = to * h[5];
+        r.M[6] = to * h[6];
+        r.M[8] = to * h[8];
+        r.M[0] = 0.f;
+        return r;
+    }
+}
+
 // x/y face populations of one finished plane, written cooperatively: edge
 // nodes park their post-collision moments in LDS (edge_m, 10 x NE values as
 // stored: FP32, or the FP16 codes that rec_setup_st decodes) and
@@ -380,7 +410,7 @@ __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__res
     float m[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) m[k] = float(edge_m[k * E::NE + tid]);
-    const Rec r = rec_setup_st<ST>(m, p);
+    const Rec r = rec_setup_st<ST>(m);
     const int x = bx * TX + tx, y = by * TY + ty;
     if (tid < 2 * TX)
         write_y_faces<Q, TX, TY, PROBE>(p, fout, r, ty, x, y, z, by);
@@ -410,35 +440,6 @@ __device__ __forceinline__ void load_plane_raw(const StepParams &p, int bx, int 
     const unsigned off = (plane_base<TX, TY>(p, bx, by, z) + (unsigned)tid) * Store<ST>::kBytes;
 #pragma unroll
     for (int k = 0; k < 10; k++) m[k] = Store<ST>::load_raw(p.mom[k], off);
-}
-
-// Population-rebuild terms from a plane loaded by load_plane_raw: FP32 as
-// rec_setup; FP16 with the decode factors folded into the products (they are
-// equal within the groups {1,2,3}, {4,7,9}, {5,6,8}: the codec's u, Pi_aa and
-// Pi_ab scales).
-template <typename ST>
-__device__ __forceinline__ Rec rec_setup_st(const float (&h)[10], const StepParams &p) {
-    if constexpr (sizeof(ST) == 4) {
-        return rec_setup(h);
-    } else {
-        Rec r;
-        const float m0 = h[0] * codec_dec(0);
-        const float rho = 1.0f + m0;
-        r.a = fmaf(rho, -(h[4] + h[7] + h[9]) * codec_dec(4), 3.0f * m0);
-        const float t = 3.0f * rho;
-        const float tu = t * codec_dec(1), td = t * codec_dec(4), to = t * codec_dec(5);
-        r.M[1] = tu * h[1];
-        r.M[2] = tu * h[2];
-        r.M[3] = tu * h[3];
-        r.M[4] = td * h[4];
-        r.M[7] = td * h[7];
-        r.M[9] = td * h[9];
-        r.M[5] = to * h[5];
-        r.M[6] = to * h[6];
-        r.M[8] = to * h[8];
-        r.M[0] = 0.f;
-        return r;
-    }
 }
 
 // Next plane's old moments as stored (load_plane_raw), or past the chunk's last
