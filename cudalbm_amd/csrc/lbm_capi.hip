@@ -206,7 +206,8 @@ struct lbm_handle {
     double omega = 0, tau = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    void *mom[10] = {};
+    void *mom[10] = {};    // the ten moment arrays, inside one allocation (mom_base)
+    void *mom_base = nullptr;
     float *faces[2] = {nullptr, nullptr};
     int64_t fwords = 0, yoff = 0, zoff = 0;
     int parity = 0;
@@ -279,10 +280,9 @@ struct lbm_handle {
     }
     void release() {
         drop_graphs();
-        for (auto &m : mom) {
-            if (m) (void)hipFree(m);
-            m = nullptr;
-        }
+        if (mom_base) (void)hipFree(mom_base);
+        mom_base = nullptr;
+        for (auto &m : mom) m = nullptr;
         for (auto &f : faces) {
             if (f) (void)hipFree(f);
             f = nullptr;
@@ -511,9 +511,16 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
             HIPX(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
             h->own_stream = true;
         }
-        for (int i = 0; i < 10; i++) {
-            HIPX(hipMalloc(&h->mom[i], h->elem() * h->n));
-            HIPX(hipMemsetAsync(h->mom[i], 0, h->elem() * h->n, h->stream));
+        {
+            // One allocation for the ten arrays, back to back (256-byte aligned),
+            // D3Q27's shifted by 4 KiB each.  Measured at 256^3 against ten
+            // separate allocations: D3Q19 FP32 265.2 -> 261.8 us, FP16 196.1 ->
+            // 193.5 us; D3Q27 FP32 395.4 -> 388.0 us with the 4 KiB shift (394.7
+            // without; D3Q19 with it: 265.9).
+            const size_t stride = (h->elem() * (size_t)h->n + 255) / 256 * 256 + (h->Q == 27 ? 4096 : 0);
+            HIPX(hipMalloc(&h->mom_base, stride * 10));
+            HIPX(hipMemsetAsync(h->mom_base, 0, stride * 10, h->stream));
+            for (int i = 0; i < 10; i++) h->mom[i] = static_cast<char *>(h->mom_base) + stride * i;
         }
         h->fwords = face_words(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY, h->LZ, &h->yoff, &h->zoff);
         if (h->fwords >= (int64_t)INT32_MAX) throw ArgError{LBM_ERR_SIZE, "face buffers exceed 2^31 words"};
