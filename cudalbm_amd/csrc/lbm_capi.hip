@@ -208,6 +208,7 @@ struct lbm_handle {
     bool own_stream = false;
     void *mom[10] = {};    // the ten moment arrays, inside one allocation (mom_base)
     void *mom_base = nullptr;
+    size_t mom_stride = 0; // bytes from one moment array to the next
     float *faces[2] = {nullptr, nullptr};
     int64_t fwords = 0, yoff = 0, zoff = 0;
     int parity = 0;
@@ -459,6 +460,34 @@ extern "C" {
 
 const char *lbm_create_error(void) { return g_create_error.c_str(); }
 
+// The ten moment arrays in one allocation, back to back (256-byte aligned),
+// each shifted by a further 4 KiB for D3Q27 and for D3Q19 with the FP64 means
+// on.  Measured at 256^3 against ten separate allocations: D3Q19 FP32 265.2
+// -> 261.8 us unshifted (265.9 shifted), FP16 196.1 -> 193.5 us; D3Q27 FP32
+// 395.4 -> 388.0 us shifted (394.7 unshifted); the D3Q19 step with the FP64
+// means 664.0 -> 657.5 us shifted (663.1 unshifted; shifting the mean arrays
+// against each other did not help).  With `keep`, the current moments move
+// into the new layout (device copies on the handle's stream).
+static void layout_moments(lbm_handle *h, unsigned fo_mask, bool keep) {
+    const bool shift = h->Q == 27 || (fo_mask & (LBM_FO_KMEAN | LBM_FO_SMEAN)) != 0;
+    const size_t stride = (h->elem() * (size_t)h->n + 255) / 256 * 256 + (shift ? 4096 : 0);
+    if (h->mom_base && stride == h->mom_stride) return;
+    void *base = nullptr;
+    HIPX(hipMalloc(&base, stride * 10));
+    if (keep && h->mom_base) {
+        for (int i = 0; i < 10; i++)
+            HIPX(hipMemcpyAsync(static_cast<char *>(base) + stride * i, h->mom[i], h->elem() * (size_t)h->n,
+                                hipMemcpyDeviceToDevice, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+        HIPX(hipFree(h->mom_base));
+    } else {
+        HIPX(hipMemsetAsync(base, 0, stride * 10, h->stream));
+    }
+    h->mom_base = base;
+    h->mom_stride = stride;
+    for (int i = 0; i < 10; i++) h->mom[i] = static_cast<char *>(base) + stride * i;
+}
+
 int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     g_create_error.clear();
     if (!cfg || !out) {
@@ -511,17 +540,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
             HIPX(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
             h->own_stream = true;
         }
-        {
-            // One allocation for the ten arrays, back to back (256-byte aligned),
-            // D3Q27's shifted by 4 KiB each.  Measured at 256^3 against ten
-            // separate allocations: D3Q19 FP32 265.2 -> 261.8 us, FP16 196.1 ->
-            // 193.5 us; D3Q27 FP32 395.4 -> 388.0 us with the 4 KiB shift (394.7
-            // without; D3Q19 with it: 265.9).
-            const size_t stride = (h->elem() * (size_t)h->n + 255) / 256 * 256 + (h->Q == 27 ? 4096 : 0);
-            HIPX(hipMalloc(&h->mom_base, stride * 10));
-            HIPX(hipMemsetAsync(h->mom_base, 0, stride * 10, h->stream));
-            for (int i = 0; i < 10; i++) h->mom[i] = static_cast<char *>(h->mom_base) + stride * i;
-        }
+        layout_moments(h, 0u, false);
         h->fwords = face_words(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY, h->LZ, &h->yoff, &h->zoff);
         if (h->fwords >= (int64_t)INT32_MAX) throw ArgError{LBM_ERR_SIZE, "face buffers exceed 2^31 words"};
         for (int s = 0; s < 2; s++) {
@@ -685,6 +704,7 @@ int lbm_fo_enable(lbm_handle *h, unsigned mask) {
         if (mask & ~15u) throw ArgError{LBM_ERR_ARG, "unknown function-object bits"};
         HIPX(hipStreamSynchronize(h->stream));
         h->drop_graphs();
+        layout_moments(h, mask, true);
         const int64_t n = h->n;
         if ((mask & LBM_FO_K) && !h->k) HIPX(hipMalloc(&h->k, sizeof(float) * n));
         if ((mask & LBM_FO_S) && !h->S) HIPX(hipMalloc(&h->S, sizeof(float) * 6 * n));

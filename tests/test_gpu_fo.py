@@ -122,3 +122,28 @@ def test_layout_tiled8_round_trip(native_lib):
     for k in range(10):
         assert np.array_equal(back_g[k], g[k])
         assert np.array_equal(back_t[k], tiled[k])
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_fo_enable_keeps_the_moments(native_lib, Q):
+    """Enabling or disabling the FP64 means may move the moment arrays to
+    another layout in device memory; the moments themselves are unchanged."""
+    from cudalbm_amd import Solver
+    from cudalbm_amd import native as N
+
+    n = 32
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=Q) as s:
+        s.step(30)
+        a = s.get_moments()
+        s.fo_enable(N.LBM_FO_KMEAN | N.LBM_FO_SMEAN)
+        b = s.get_moments()
+        s.step(5)
+        s.fo_enable(0)
+        c = s.get_moments()
+        s.step(5)
+        d = s.get_moments()
+    print(f"D3Q{Q}: moments kept across both layout changes", flush=True)
+    for k in range(10):
+        assert np.array_equal(a[k], b[k]), k
+    assert all(np.isfinite(np.asarray(d[k])).all() for k in range(10))
+    assert not np.array_equal(c[1], d[1])
