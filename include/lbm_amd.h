@@ -116,7 +116,9 @@ int lbm_set_time(lbm_handle *h, int64_t t);
  * (kineticEnergy.cuh:249).  The instantaneous fields (k, S) are functions of
  * the current moments alone, so they are formed from them when read, with the
  * same formulas (kineticEnergy.cuh:71-85, strainRateTensor.cuh:71-84).
- * Enabling resets the means to zero. */
+ * Enabling resets the means to zero.  Synchronizes the handle's stream; turning
+ * the means on or off may move the moment arrays inside device memory (a
+ * layout measured faster for the step with the means), keeping their values. */
 int lbm_fo_enable(lbm_handle *h, unsigned mask);
 /* Reads an FO field.  LBM_FIELD_K/KMEAN: n values; S/SMEAN: 6*n values in the
  * order xx,xy,xz,yy,yz,zz, component-major. */
