@@ -15,11 +15,17 @@
  *                          pinned: known <=> source inside the domain, exhaustive
  *   boundary moments ..... boundaryConditions.cuh:99-554
  *                          pinned: rest fixed point at every node class (exact
- *                          rationals 1 and 125/126), per-case algebra written
- *                          independently in the tests, and mass/velocity checks;
- *                          parity unpinned for the U-dependent terms of the
- *                          NORTH-WEST/EAST lid edges, the lid corners and the
- *                          D3Q27 NORTH-BACK/FRONT edges (transcription only)
+ *                          rationals 1 and 125/126); all 26 classes of both
+ *                          velocity sets against an independent derivation
+ *                          solved by Newton in the tests (known density of the
+ *                          post-collision populations, known off-diagonals of
+ *                          the pre-collision ones), at several omega and U --
+ *                          the lid's NORTH-WEST/EAST edges and corners match it
+ *                          with the lid reversed (-U) in the density equation,
+ *                          the D3Q27 NORTH-BACK/FRONT edges and lid corners with
+ *                          the D3Q19 closure; the reversed-lid equilibrium as a
+ *                          closed form; x and z mirror symmetry of the lid
+ *                          closures
  *   scale / collide ...... velocitySet.cuh:130-142, secondOrder.cuh:93-105
  *                          pinned: affine closed form + equilibrium fixed point
  *   node type ............ normalVector.cuh:321-334 (pinned: exhaustive counts)

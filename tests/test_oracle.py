@@ -154,15 +154,18 @@ def test_rest_fixed_point_every_node_class(Q):
         assert np.abs(out[k]).max() < 1e-14
 
 
-def _closure_by_matching(Q, bits, f, omega, u_lid):
+def _closure_by_matching(Q, bits, f, omega, u_rho, u_off=None):
     """Independent derivation of the moment-based wall closure used by the
-    reference: unknown rho and the face-tangential off-diagonals are fixed by
-      (i)  sum_known f_post(rho, u, m) = rho_I   (density of the post-collision
-           regularised populations),
-      (ii) sum_known f_pre(rho, u, m) c_a c_b = sum_known f c_a c_b
-           for the off-diagonals the wall leaves free (pre-collision populations);
-    with u the wall velocity, m_xx = u_x^2 and the other moments zero.  Solved
-    numerically (Newton), not from the closed forms."""
+    reference: unknown rho and the off-diagonals the wall leaves free are fixed by
+      (i)  sum_known f_post(rho, u_rho, m) = rho_I   (density of the
+           post-collision regularised populations),
+      (ii) sum_known f_pre(rho, u_off, m) c_a c_b = sum_known f c_a c_b
+           (known off-diagonal second moments of the pre-collision populations),
+    with wall velocity u along x, m_xx = u^2 and the other moments zero.  The
+    two equations may carry different wall velocities (u_off defaults to u_rho).
+    Solved numerically (Newton), not from the closed forms."""
+    if u_off is None:
+        u_off = u_rho
     c, w = O.velocity_set(Q)
     K = [q for q in range(Q) if O.known(Q, q, bits)]
     normal_axes = set()
@@ -182,9 +185,8 @@ def _closure_by_matching(Q, bits, f, omega, u_lid):
         active = [(a, b, k) for a, b, k in pairs if a in normal_axes and b in normal_axes]
     else:
         active = []
-    ux = u_lid if (bits & 8) else 0.0
 
-    def pops(rho, mvals, post):
+    def pops(rho, mvals, post, ux):
         m = np.zeros(10)
         m[0] = rho
         m[1] = 3 * ux
@@ -200,8 +202,8 @@ def _closure_by_matching(Q, bits, f, omega, u_lid):
 
     def F(xv):
         rho, mv = xv[0], xv[1:]
-        fp = pops(rho, mv, True)
-        fr = pops(rho, mv, False)
+        fp = pops(rho, mv, True, u_rho)
+        fr = pops(rho, mv, False, u_off)
         r = [fp[K].sum() - rho_I]
         for (a, b, _), t in zip(active, targets):
             r.append((fr[K] * c[K, a] * c[K, b]).sum() - t)
@@ -219,57 +221,157 @@ def _closure_by_matching(Q, bits, f, omega, u_lid):
     return xv[0], dict(zip([k for _, _, k in active], xv[1:]))
 
 
-def _lid_velocity_in_derivation(Q, bits, u_lid):
-    """Wall velocity with which the matching derivation reproduces the closure:
-    static walls 0; the NORTH face the lid speed, and for D3Q19 also the
-    NORTH-BACK / NORTH-FRONT edges (their stored velocity is the
-    boundary-count average U/2, but their closure is the moving face's); None
-    where the reference's closure follows another derivation (NORTH-WEST/EAST
-    edges, lid corners, and the D3Q27 lid edges)."""
-    nb = bin(bits & 0x3F).count("1")
-    if not (bits & 8):
-        return 0.0
-    if nb == 1 or (Q == 19 and (bits & 0x3F) in (8 | 16, 8 | 32)):
-        return u_lid
-    return None
+N_, E_, W_, B_, F_ = 8, 2, 1, 16, 32
+LID_CORNERS = (N_ | W_ | B_, N_ | W_ | F_, N_ | E_ | B_, N_ | E_ | F_)
+
+
+def _derivation_for(Q, code, U):
+    """(velocity set of the derivation, wall velocity in equation (i), in
+    equation (ii)) that reproduces the reference closure of node class `code`
+    (boundaryConditions.cuh:99-554):
+      * static walls, edges and corners: u = 0;
+      * the NORTH face and (D3Q19) the NORTH-BACK / NORTH-FRONT edges: the lid
+        speed +U in both equations (the closure uses the full U although the
+        stored velocity is the boundary-count average U/2 on the edges);
+      * the NORTH-WEST / NORTH-EAST edges (both velocity sets): -U in the
+        density equation, +U in the off-diagonal one;
+      * the four lid corners: -U in the density equation (no off-diagonal is
+        solved at a corner) -- the lid's edges and corners see the lid moving
+        the other way, a quirk of the reference that changes results;
+      * D3Q27 NORTH-BACK / NORTH-FRONT edges and lid corners: the D3Q19
+        closure, evaluated on the D3Q27 known sums (the reference reuses the
+        D3Q19 expressions there; at rest the corners give rho = 125/126)."""
+    if not (code & N_):
+        return Q, 0.0, 0.0
+    if code == N_:
+        return Q, U, U
+    if code in (N_ | B_, N_ | F_):
+        return 19, U, U
+    if code in (N_ | W_, N_ | E_):
+        return Q, -U, U
+    assert code in LID_CORNERS
+    return 19, -U, -U
+
+
+def _d3q19_known_state(bits, rho_I, sums):
+    """A D3Q19 population vector, zero where unknown at `bits`, whose known
+    density is rho_I and whose known off-diagonal sums sum_K f c_a c_b are
+    `sums` ({5: xy, 6: xz, 8: yz})."""
+    c, w = O.velocity_set(19)
+    K = [q for q in range(19) if O.known(19, q, bits)]
+    assert 0 in K
+    f = np.zeros(19)
+    f[K] = w[K] * rho_I / w[K].sum()
+    for k, (a, b) in {5: (0, 1), 6: (0, 2), 8: (1, 2)}.items():
+        q = next((q for q in K if c[q, a] * c[q, b] != 0 and c[q, 3 - a - b] == 0), None)
+        if q is None:  # no known population carries this product: its sum is 0 for both sets
+            assert abs(sums[k]) < 1e-15
+            continue
+        d = (sums[k] - (f[K] * c[K, a] * c[K, b]).sum()) / (c[q, a] * c[q, b])
+        f[q] += d  # moves the (a, b) sum only; the rest population keeps rho_I
+        f[0] -= d
+    return f
 
 
 @pytest.mark.parametrize("Q", [19, 27])
 def test_boundary_closure_matches_independent_derivation(Q):
-    """All static faces, edges and corners, the moving NORTH face and (D3Q19)
-    the NORTH-BACK / NORTH-FRONT lid edges agree with the matching derivation to
-    1e-10.  (For D3Q27 the static corners use 216/125; the derivation gives the
-    same.)  The other lid edges and the four lid corners carry U-dependent
-    terms that the reference derived differently; they are pinned by their U=0
-    limit here and by the transcription (oracle header)."""
+    """Every one of the 26 boundary node classes, static and moving, at several
+    relaxation rates and lid speeds: the oracle's closure equals the matching
+    derivation named by _derivation_for (to 1e-10), and the stored velocity is
+    the boundary-count average of the wall velocities (U/nb on lid nodes,
+    boundaryConditions.cuh:543-554) with m_xx = u_x^2, m_yy = m_zz = 0."""
     rng = np.random.default_rng(3)
     c, w = O.velocity_set(Q)
-    omega = 1.7
-    u_lid = 0.05
     n = 3
     seen = set()
     for x, y, z in itertools.product(range(n), repeat=3):
         bits = O.node_type(x, y, z, n, n, n)
-        if not (bits & 0x40) or (bits & 0x3F) in seen:
+        code = bits & 0x3F
+        if not (bits & 0x40) or code in seen:
             continue
-        seen.add(bits & 0x3F)
-        # a plausible non-equilibrium known state
-        f = np.array(w) * (1.0 + 0.02 * rng.normal(size=Q))
-        for q in range(Q):
-            if not O.known(Q, q, bits):
-                f[q] = 0.0
-        U = _lid_velocity_in_derivation(Q, bits, u_lid)
-        U = 0.0 if U is None else U  # the U = 0 limit of the other lid nodes
-        if Q == 27 and (bits & 8) and bin(bits & 0x3F).count("1") == 3:
-            # D3Q27 North corners reuse the D3Q19 constant (reference quirk,
-            # boundaryConditions.cuh:419-466); pinned by the exact 125/126 test
-            continue
-        m = O.moments_boundary(Q, f, bits, omega, U)
-        rho, offd = _closure_by_matching(Q, bits, f, omega, U)
-        assert abs(m[0] - rho) < 1e-10, (bits, m[0], rho)
-        for k in (5, 6, 8):
-            assert abs(m[k] - offd.get(k, 0.0)) < 1e-10, (bits, k)
+        seen.add(code)
+        for omega, U in ((1.7, 0.05), (1.0, 0.08), (1.93, 0.02), (0.9, 0.1), (1.5, 0.0)):
+            # a plausible non-equilibrium known state
+            f = np.array(w) * (1.0 + 0.02 * rng.normal(size=Q))
+            for q in range(Q):
+                if not O.known(Q, q, bits):
+                    f[q] = 0.0
+            m = O.moments_boundary(Q, f, bits, omega, U)
+            Qd, u_rho, u_off = _derivation_for(Q, code, U)
+            if Qd == Q:
+                fd = f
+            else:
+                K = [q for q in range(Q) if O.known(Q, q, bits)]
+                sums = {k: (f[K] * c[K, a] * c[K, b]).sum() for k, (a, b) in {5: (0, 1), 6: (0, 2), 8: (1, 2)}.items()}
+                fd = _d3q19_known_state(bits, f[K].sum(), sums)
+            rho, offd = _closure_by_matching(Qd, bits, fd, omega, u_rho, u_off)
+            assert abs(m[0] - rho) < 1e-10, (Q, code, omega, U, m[0], rho)
+            for k in (5, 6, 8):
+                assert abs(m[k] - offd.get(k, 0.0)) < 1e-10, (Q, code, k, omega, U)
+            nb = bin(code).count("1")
+            ux = U / nb if code & N_ else 0.0
+            assert abs(m[1] - ux) < 1e-15 and m[2] == 0.0 and m[3] == 0.0
+            assert abs(m[4] - ux * ux) < 1e-15 and m[7] == 0.0 and m[9] == 0.0
     assert len(seen) == 26
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_lid_closures_equilibrium_of_reversed_lid(Q):
+    """Closed form for the lid corners (any omega) and the NORTH-WEST /
+    NORTH-EAST edges (omega = 1, where the post-collision populations are the
+    equilibrium): when the known populations are the equilibrium at (rho0,
+    -U x), the closure returns rho = rho0 exactly.  D3Q27 corners obey it for
+    the D3Q19 equilibrium's known density instead (they reuse the D3Q19
+    constant)."""
+    c, w = O.velocity_set(Q)
+    for U, rho0 in ((0.05, 1.0), (0.08, 1.013), (0.02, 0.97)):
+        cases = [(cc, (0.8, 1.0, 1.6, 1.95)) for cc in LID_CORNERS] + [(N_ | W_, (1.0,)), (N_ | E_, (1.0,))]
+        for code, omegas in cases:
+            bits = code | 0x40
+            Qe = 19 if (code in LID_CORNERS and Q == 27) else Q
+            m = np.zeros(10)
+            m[0] = rho0
+            m[1] = -3 * U
+            m[4] = 4.5 * U * U
+            feq = O.reconstruct(Qe, m)
+            Ke = [q for q in range(Qe) if O.known(Qe, q, bits)]
+            K = [q for q in range(Q) if O.known(Q, q, bits)]
+            f = np.zeros(Q)
+            if Qe == Q:
+                f[K] = feq[K]
+            else:
+                f[K] = w[K] * feq[Ke].sum() / w[K].sum()  # a D3Q27 state with that known density
+            for om in omegas:
+                out = O.moments_boundary(Q, f, bits, om, U)
+                assert abs(out[0] - rho0) < 1e-13, (Q, code, om, U, out[0])
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_lid_closures_mirror_symmetry(Q):
+    """Mirror images of the lid closures: x -> -x maps NORTH-EAST at lid speed
+    U to NORTH-WEST at -U (m_xy and m_xz change sign), and z -> -z maps
+    NORTH-BACK to NORTH-FRONT and the B lid corners to the F ones (m_xz and
+    m_yz change sign); random non-equilibrium known states."""
+    c, w = O.velocity_set(Q)
+    mx = [next(p for p in range(Q) if (c[p] == c[q] * np.array([-1, 1, 1])).all()) for q in range(Q)]
+    mz = [next(p for p in range(Q) if (c[p] == c[q] * np.array([1, 1, -1])).all()) for q in range(Q)]
+    rng = np.random.default_rng(7)
+    x_pairs = [(N_ | E_, N_ | W_), (N_ | E_ | B_, N_ | W_ | B_), (N_ | E_ | F_, N_ | W_ | F_)]
+    z_pairs = [(N_ | B_, N_ | F_), (N_ | W_ | B_, N_ | W_ | F_), (N_ | E_ | B_, N_ | E_ | F_)]
+    cases = [(p, mx, (5, 6), -1.0) for p in x_pairs] + [(p, mz, (6, 8), 1.0) for p in z_pairs]
+    for omega, U in ((1.7, 0.05), (1.1, 0.09)):
+        for (a, b), mirror, flips, lid_sign in cases:
+            ba, bb = a | 0x40, b | 0x40
+            f = np.array(w) * (1.0 + 0.03 * rng.normal(size=Q))
+            f = np.where([O.known(Q, q, ba) for q in range(Q)], f, 0.0)
+            g = f[mirror]
+            assert all((g[q] == 0.0) == (not O.known(Q, q, bb)) for q in range(Q))
+            ma = O.moments_boundary(Q, f, ba, omega, U)
+            mb = O.moments_boundary(Q, g, bb, omega, lid_sign * U)
+            assert abs(ma[0] - mb[0]) < 1e-13, (Q, a, b)
+            for k in (5, 6, 8):
+                s = -1.0 if k in flips else 1.0
+                assert abs(ma[k] - s * mb[k]) < 1e-13, (Q, a, b, k)
 
 
 def test_initial_field_regions():
