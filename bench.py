@@ -6,8 +6,8 @@ Workload: lid-driven cavity, D3Q19, FP32 storage and arithmetic, 256^3 nodes,
 Re 1000, lid speed 0.05, started from rest, with the function objects k, kMean,
 S and SMean on.  One "step" is one full LBM update of every node together with
 that step's function-object work (one launch of the fused step kernel, which
-updates the FP64 time means, plus the one-thread launch that advances the step
-counter the means divide by); the timed region holds K steps on the solver's
+updates the FP64 time means and advances the device step counter they divide
+by); the timed region holds K steps on the solver's
 HIP stream, bracketed by hipEvents, with all state resident in HBM.  Prints ONE
 JSON line.
 

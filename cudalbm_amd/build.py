@@ -82,13 +82,6 @@ def build_apps(verbose: bool = False) -> None:
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-    # HBM copy-rate probe (no library dependence)
-    cmd = ["hipcc", *HIPCC_FLAGS, os.path.join(apps, "hbm_copy_probe.hip"), "-o", os.path.join(BIN, "hbmCopyProbe")]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hbmCopyProbe build failed:\n" + r.stderr)
 
 
 if __name__ == "__main__":

@@ -24,6 +24,9 @@ namespace {
 const char *kNames[10] = {"rho", "u", "v", "w", "m_xx", "m_xy", "m_xz", "m_yy", "m_yz", "m_zz"};
 const char *kS[6] = {"S_xx", "S_xy", "S_xz", "S_yy", "S_yz", "S_zz"};
 const char *kSMean[6] = {"S_xxMean", "S_xyMean", "S_xzMean", "S_yyMean", "S_yzMean", "S_zzMean"};
+// functionObjects.cuh:66
+const char *kMomentsMean[10] = {"rhoMean", "uMean", "vMean", "wMean", "m_xxMean",
+                                "m_xyMean", "m_xzMean", "m_yyMean", "m_yzMean", "m_zzMean"};
 
 [[noreturn]] void die(int code, const std::string &msg) {
     std::fprintf(stderr, "%s\n", msg.c_str());
@@ -65,7 +68,7 @@ void save_fo(lbm_handle *h, const lbm_case_info &ci, unsigned mask, int64_t t) {
         std::vector<double> d(n * ncomp);
         check(h, lbm_fo_get(h, field, d.data(), LBM_LAYOUT_GLOBAL), "lbm_fo_get");
         std::vector<std::vector<float>> f(ncomp, std::vector<float>(n));
-        const float *p[6];
+        const float *p[10];
         for (int c = 0; c < ncomp; c++) {
             for (size_t i = 0; i < n; i++) f[c][i] = (float)d[c * n + i];
             p[c] = f[c].data();
@@ -79,6 +82,7 @@ void save_fo(lbm_handle *h, const lbm_case_info &ci, unsigned mask, int64_t t) {
     if (mask & LBM_FO_KMEAN) write("kMean", LBM_FIELD_KMEAN, 1, km, 1);
     if (mask & LBM_FO_S) write("S", LBM_FIELD_S, 6, kS, 0);
     if (mask & LBM_FO_SMEAN) write("SMean", LBM_FIELD_SMEAN, 6, kSMean, 1);
+    if (mask & LBM_FO_MOMENTSMEAN) write("momentsMean", LBM_FIELD_MOMENTSMEAN, 10, kMomentsMean, 1);
 }
 
 }  // namespace
@@ -138,8 +142,6 @@ int run_moment_based(int Q, int argc, char **argv) {
 
     unsigned fo = 0;
     if (lbm_case_function_objects(".", &fo)) die(1, lbm_case_last_error());
-    if (fo & 16u) std::printf("note: momentsMean is not computed by this solver\n");
-    fo &= 15u;
     if (fo) check(h, lbm_fo_enable(h, fo), "lbm_fo_enable");
 
     const auto t0 = std::chrono::steady_clock::now();

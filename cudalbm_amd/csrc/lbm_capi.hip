@@ -199,6 +199,20 @@ std::vector<int64_t> make_perm(int layout, int nx, int ny, int nz, int tx, int t
 
 }  // namespace lbm
 
+// Function objects whose FP64 running means are fused into the step.
+constexpr unsigned kMeanBits = LBM_FO_KMEAN | LBM_FO_SMEAN | LBM_FO_MOMENTSMEAN;
+
+// Device allocation whose out-of-memory failure is reported as LBM_ERR_NOMEM.
+static void dev_alloc(void **p, size_t bytes, const char *what) {
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) return;
+    (void)hipGetLastError();
+    *p = nullptr;
+    if (e == hipErrorOutOfMemory)
+        throw ArgError{LBM_ERR_NOMEM, std::string("device out of memory allocating ") + what};
+    throw HipError{e, what};
+}
+
 struct lbm_handle {
     lbm_config cfg{};
     int Q = 19, TX = 16, TY = 16, LZ = 16, nbx = 0, nby = 0, nc = 0;
@@ -214,7 +228,7 @@ struct lbm_handle {
     int parity = 0;
     unsigned fo_mask = 0;
     float *k = nullptr, *S = nullptr;
-    double *kmean = nullptr, *smean = nullptr;
+    double *kmean = nullptr, *smean = nullptr, *mmean = nullptr;
     StepClock *clock = nullptr;
     int64_t t = 0;
     float *uprev = nullptr;
@@ -255,6 +269,7 @@ struct lbm_handle {
         p.fo_mask = probe ? 0u : fo_mask;
         p.kmean = kmean;
         p.smean = smean;
+        p.mmean = mmean;
         p.clock = clock + parity;
         p.clock_next = clock + (parity ^ 1);
         p.s_diag = (float)(1.0 / (3.0 * tau));
@@ -272,7 +287,7 @@ struct lbm_handle {
     }
     int grid() const { return nbx * nby * nc; }
     // the step carries the FP64 time means (instantaneous k/S are formed on read)
-    bool means() const { return (fo_mask & (LBM_FO_KMEAN | LBM_FO_SMEAN)) != 0; }
+    bool means() const { return (fo_mask & kMeanBits) != 0; }
     size_t elem() const { return cfg.storage == LBM_STORE_FP16 ? 2 : 4; }
 
     void drop_graphs() {
@@ -288,11 +303,11 @@ struct lbm_handle {
             if (f) (void)hipFree(f);
             f = nullptr;
         }
-        for (void *p : {(void *)k, (void *)S, (void *)kmean, (void *)smean, (void *)clock, (void *)uprev,
-                        (void *)part, (void *)red_out})
+        for (void *p : {(void *)k, (void *)S, (void *)kmean, (void *)smean, (void *)mmean, (void *)clock,
+                        (void *)uprev, (void *)part, (void *)red_out})
             if (p) (void)hipFree(p);
         k = S = nullptr;
-        kmean = smean = nullptr;
+        kmean = smean = mmean = nullptr;
         clock = nullptr;
         uprev = nullptr;
         part = red_out = nullptr;
@@ -467,22 +482,36 @@ const char *lbm_create_error(void) { return g_create_error.c_str(); }
 // 395.4 -> 388.0 us shifted (394.7 unshifted); the D3Q19 step with the FP64
 // means 664.0 -> 657.5 us shifted (663.1 unshifted; shifting the mean arrays
 // against each other did not help).  With `keep`, the current moments move
-// into the new layout (device copies on the handle's stream).
+// into the new layout (device copies on the handle's stream): the old and the
+// new block are both held until the copy has run.  The shift only changes
+// speed, so a move that cannot get its new block keeps the current layout.
 static void layout_moments(lbm_handle *h, unsigned fo_mask, bool keep) {
-    const bool shift = h->Q == 27 || (fo_mask & (LBM_FO_KMEAN | LBM_FO_SMEAN)) != 0;
+    const bool shift = h->Q == 27 || (fo_mask & kMeanBits) != 0;
     const size_t stride = (h->elem() * (size_t)h->n + 255) / 256 * 256 + (shift ? 4096 : 0);
     if (h->mom_base && stride == h->mom_stride) return;
     void *base = nullptr;
-    HIPX(hipMalloc(&base, stride * 10));
-    if (keep && h->mom_base) {
-        for (int i = 0; i < 10; i++)
-            HIPX(hipMemcpyAsync(static_cast<char *>(base) + stride * i, h->mom[i], h->elem() * (size_t)h->n,
-                                hipMemcpyDeviceToDevice, h->stream));
-        HIPX(hipStreamSynchronize(h->stream));
-        HIPX(hipFree(h->mom_base));
-    } else {
-        HIPX(hipMemsetAsync(base, 0, stride * 10, h->stream));
+    const hipError_t e = hipMalloc(&base, stride * 10);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (e == hipErrorOutOfMemory && keep && h->mom_base) return;
+        if (e == hipErrorOutOfMemory) throw ArgError{LBM_ERR_NOMEM, "device out of memory allocating the moments"};
+        throw HipError{e, "hipMalloc(moments)"};
     }
+    try {
+        if (keep && h->mom_base) {
+            for (int i = 0; i < 10; i++)
+                HIPX(hipMemcpyAsync(static_cast<char *>(base) + stride * i, h->mom[i], h->elem() * (size_t)h->n,
+                                    hipMemcpyDeviceToDevice, h->stream));
+            HIPX(hipStreamSynchronize(h->stream));
+        } else {
+            HIPX(hipMemsetAsync(base, 0, stride * 10, h->stream));
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipFree(base);
+        throw;
+    }
+    if (h->mom_base) (void)hipFree(h->mom_base);
     h->mom_base = base;
     h->mom_stride = stride;
     for (int i = 0; i < 10; i++) h->mom[i] = static_cast<char *>(base) + stride * i;
@@ -544,10 +573,10 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
         h->fwords = face_words(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY, h->LZ, &h->yoff, &h->zoff);
         if (h->fwords >= (int64_t)INT32_MAX) throw ArgError{LBM_ERR_SIZE, "face buffers exceed 2^31 words"};
         for (int s = 0; s < 2; s++) {
-            HIPX(hipMalloc(&h->faces[s], sizeof(float) * h->fwords));
+            dev_alloc((void **)&h->faces[s], sizeof(float) * h->fwords, "face buffers");
             HIPX(hipMemsetAsync(h->faces[s], 0, sizeof(float) * h->fwords, h->stream));
         }
-        HIPX(hipMalloc(&h->clock, 2 * sizeof(StepClock)));
+        dev_alloc((void **)&h->clock, 2 * sizeof(StepClock), "step clock");
         set_clock(h);
         rebuild_faces(h);
         HIPX(hipStreamSynchronize(h->stream));
@@ -701,23 +730,25 @@ int lbm_set_time(lbm_handle *h, int64_t t) {
 
 int lbm_fo_enable(lbm_handle *h, unsigned mask) {
     return guarded(h, [&] {
-        if (mask & ~15u) throw ArgError{LBM_ERR_ARG, "unknown function-object bits"};
+        if (mask & ~31u) throw ArgError{LBM_ERR_ARG, "unknown function-object bits"};
         HIPX(hipStreamSynchronize(h->stream));
         h->drop_graphs();
         layout_moments(h, mask, true);
         const int64_t n = h->n;
-        if ((mask & LBM_FO_K) && !h->k) HIPX(hipMalloc(&h->k, sizeof(float) * n));
-        if ((mask & LBM_FO_S) && !h->S) HIPX(hipMalloc(&h->S, sizeof(float) * 6 * n));
-        if ((mask & LBM_FO_KMEAN) && !h->kmean) HIPX(hipMalloc(&h->kmean, sizeof(double) * n));
-        if ((mask & LBM_FO_SMEAN) && !h->smean) HIPX(hipMalloc(&h->smean, sizeof(double) * 6 * n));
-        if (mask & LBM_FO_KMEAN) {
-            hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, h->kmean, n, 0.0);
-            HIPX(hipGetLastError());
-        }
-        if (mask & LBM_FO_SMEAN) {
-            hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, h->smean, 6 * n, 0.0);
-            HIPX(hipGetLastError());
-        }
+        if ((mask & LBM_FO_K) && !h->k) dev_alloc((void **)&h->k, sizeof(float) * n, "k");
+        if ((mask & LBM_FO_S) && !h->S) dev_alloc((void **)&h->S, sizeof(float) * 6 * n, "S");
+        if ((mask & LBM_FO_KMEAN) && !h->kmean) dev_alloc((void **)&h->kmean, sizeof(double) * n, "kMean");
+        if ((mask & LBM_FO_SMEAN) && !h->smean) dev_alloc((void **)&h->smean, sizeof(double) * 6 * n, "SMean");
+        if ((mask & LBM_FO_MOMENTSMEAN) && !h->mmean)
+            dev_alloc((void **)&h->mmean, sizeof(double) * 10 * n, "momentsMean");
+        const std::pair<unsigned, std::pair<double *, int64_t>> zero[3] = {
+            {LBM_FO_KMEAN, {h->kmean, n}}, {LBM_FO_SMEAN, {h->smean, 6 * n}}, {LBM_FO_MOMENTSMEAN, {h->mmean, 10 * n}}};
+        for (const auto &z : zero)
+            if (mask & z.first) {
+                hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, z.second.first,
+                                   z.second.second, 0.0);
+                HIPX(hipGetLastError());
+            }
         set_clock(h);
         HIPX(hipStreamSynchronize(h->stream));
         h->fo_mask = mask;
@@ -731,11 +762,13 @@ int lbm_fo_get(lbm_handle *h, int field, double *out, int layout) {
         const unsigned need = field == LBM_FIELD_K ? LBM_FO_K
                               : field == LBM_FIELD_KMEAN ? LBM_FO_KMEAN
                               : field == LBM_FIELD_S ? LBM_FO_S
-                              : field == LBM_FIELD_SMEAN ? LBM_FO_SMEAN : 0u;
+                              : field == LBM_FIELD_SMEAN ? LBM_FO_SMEAN
+                              : field == LBM_FIELD_MOMENTSMEAN ? LBM_FO_MOMENTSMEAN : 0u;
         if (!need) throw ArgError{LBM_ERR_ARG, "unknown field"};
         if (!(h->fo_mask & need)) throw ArgError{LBM_ERR_STATE, "function object not enabled"};
         HIPX(hipStreamSynchronize(h->stream));
-        const int ncomp = (field == LBM_FIELD_S || field == LBM_FIELD_SMEAN) ? 6 : 1;
+        const int ncomp = (field == LBM_FIELD_S || field == LBM_FIELD_SMEAN) ? 6
+                          : field == LBM_FIELD_MOMENTSMEAN ? 10 : 1;
         const int64_t n = h->n;
         const auto &perm = h->perm(layout);
         std::vector<double> tmp((size_t)(n * ncomp));
@@ -754,8 +787,8 @@ int lbm_fo_get(lbm_handle *h, int field, double *out, int layout) {
                            hipMemcpyDeviceToHost));
             for (size_t i = 0; i < f.size(); i++) tmp[i] = f[i];
         } else {
-            HIPX(hipMemcpy(tmp.data(), field == LBM_FIELD_KMEAN ? h->kmean : h->smean, sizeof(double) * n * ncomp,
-                           hipMemcpyDeviceToHost));
+            const double *src = field == LBM_FIELD_KMEAN ? h->kmean : field == LBM_FIELD_SMEAN ? h->smean : h->mmean;
+            HIPX(hipMemcpy(tmp.data(), src, sizeof(double) * n * ncomp, hipMemcpyDeviceToHost));
         }
         for (int c = 0; c < ncomp; c++)
             for (int64_t u = 0; u < n; u++) out[c * n + u] = tmp[(size_t)(c * n + perm[(size_t)u])];
@@ -764,10 +797,10 @@ int lbm_fo_get(lbm_handle *h, int field, double *out, int layout) {
 
 int lbm_reduce(lbm_handle *h, double *ke, double *res) {
     return guarded(h, [&] {
-        if (!h->uprev) {
-            HIPX(hipMalloc(&h->uprev, sizeof(float) * 3 * h->n));
-            HIPX(hipMalloc(&h->part, sizeof(double) * 3 * h->red_blocks));
-            HIPX(hipMalloc(&h->red_out, sizeof(double) * 3));
+        if (!h->uprev || !h->part || !h->red_out) {  // first call, or a first call that failed part-way
+            if (!h->uprev) dev_alloc((void **)&h->uprev, sizeof(float) * 3 * h->n, "previous velocity");
+            if (!h->part) dev_alloc((void **)&h->part, sizeof(double) * 3 * h->red_blocks, "reduction partials");
+            if (!h->red_out) dev_alloc((void **)&h->red_out, sizeof(double) * 3, "reduction result");
             h->red_first = true;
         }
         StepParams p = h->params();

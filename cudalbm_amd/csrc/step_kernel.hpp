@@ -54,10 +54,12 @@ struct StepParams {
     float *fout;           // face buffers written (this step)
     int yoff, zoff;        // offsets of the Y and Z face blocks inside one face set
     float enc[10], dec[10];  // codec_enc/codec_dec (FP16) or 1 (FP32), for the auxiliary kernels
-    // fused FP64 time means of the function objects (fo_mask bits 2: kMean, 8: SMean)
+    // fused FP64 time means of the function objects (fo_mask bits 2: kMean,
+    // 8: SMean, 16: momentsMean)
     unsigned fo_mask;
     double *kmean;
     double *smean;         // 6 component arrays, each of n doubles
+    double *mmean;         // 10 component arrays (means of the stored moments), each of n doubles
     const StepClock *clock; // absolute step index of this step and 1/(t+1) (device)
     StepClock *clock_next;  // the next step's clock, or null when nothing reads it
     float s_diag, s_off;    // strain-rate coefficients 1/(3 tau), 1/(6 tau)

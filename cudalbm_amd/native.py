@@ -17,8 +17,8 @@ LBM_STORE_FP32 = 0
 LBM_STORE_FP16 = 1
 LBM_LAYOUT_GLOBAL = 0
 LBM_LAYOUT_TILED8 = 1
-LBM_FO_K, LBM_FO_KMEAN, LBM_FO_S, LBM_FO_SMEAN = 1, 2, 4, 8
-LBM_FIELD_K, LBM_FIELD_KMEAN, LBM_FIELD_S, LBM_FIELD_SMEAN = 0, 1, 2, 3
+LBM_FO_K, LBM_FO_KMEAN, LBM_FO_S, LBM_FO_SMEAN, LBM_FO_MOMENTSMEAN = 1, 2, 4, 8, 16
+LBM_FIELD_K, LBM_FIELD_KMEAN, LBM_FIELD_S, LBM_FIELD_SMEAN, LBM_FIELD_MOMENTSMEAN = 0, 1, 2, 3, 4
 
 # Every symbol include/lbm_amd.h declares (checked by tests/test_native_abi.py).
 EXPORTED = [
@@ -198,7 +198,7 @@ class Solver:
         self._check(lib().lbm_fo_enable(self._h, int(mask)))
 
     def fo_get(self, field, layout=LBM_LAYOUT_GLOBAL):
-        ncomp = 6 if field in (LBM_FIELD_S, LBM_FIELD_SMEAN) else 1
+        ncomp = 6 if field in (LBM_FIELD_S, LBM_FIELD_SMEAN) else (10 if field == LBM_FIELD_MOMENTSMEAN else 1)
         out = np.empty(ncomp * self.n, dtype=np.float64)
         self._check(lib().lbm_fo_get(self._h, field, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), layout))
         return out.reshape(ncomp, self.n) if ncomp > 1 else out

@@ -55,11 +55,19 @@ enum {
     LBM_FO_K = 1u << 0,     /* k, instantaneous (FP32) */
     LBM_FO_KMEAN = 1u << 1, /* kMean, running mean (FP64 accumulator) */
     LBM_FO_S = 1u << 2,     /* S_xx S_xy S_xz S_yy S_yz S_zz, instantaneous (FP32) */
-    LBM_FO_SMEAN = 1u << 3  /* SMean, running means (FP64 accumulators) */
+    LBM_FO_SMEAN = 1u << 3, /* SMean, running means (FP64 accumulators) */
+    LBM_FO_MOMENTSMEAN = 1u << 4 /* momentsMean: running means of the 10 stored moments
+                                    (FP64 accumulators; src/functionObjects/moments.cuh:70-101) */
 };
 
 /* Which FO field to read back with lbm_fo_get. */
-typedef enum { LBM_FIELD_K = 0, LBM_FIELD_KMEAN = 1, LBM_FIELD_S = 2, LBM_FIELD_SMEAN = 3 } lbm_fo_field;
+typedef enum {
+    LBM_FIELD_K = 0,
+    LBM_FIELD_KMEAN = 1,
+    LBM_FIELD_S = 2,
+    LBM_FIELD_SMEAN = 3,
+    LBM_FIELD_MOMENTSMEAN = 4
+} lbm_fo_field;
 
 typedef struct lbm_config {
     int velocity_set; /* 19 or 27 */
@@ -110,18 +118,23 @@ int lbm_get_time(const lbm_handle *h, int64_t *t);
 int lbm_set_time(lbm_handle *h, int64_t t);
 
 /* Replaces objectRegistry (objectRegistry.cuh:68-117), whose calculate() runs
- * after every step (momentBasedD3Q19.cu:133).  The time means (kMean, SMean)
- * are fused into the step: FP64 accumulators updated from the stored
+ * after every step (momentBasedD3Q19.cu:133).  The time means (kMean, SMean,
+ * momentsMean) are fused into the step: FP64 accumulators updated from the stored
  * post-collision moments with invNewCount = 1/(t+1) of the absolute step t
  * (kineticEnergy.cuh:249).  The instantaneous fields (k, S) are functions of
  * the current moments alone, so they are formed from them when read, with the
  * same formulas (kineticEnergy.cuh:71-85, strainRateTensor.cuh:71-84).
  * Enabling resets the means to zero.  Synchronizes the handle's stream; turning
  * the means on or off may move the moment arrays inside device memory (a
- * layout measured faster for the step with the means), keeping their values. */
+ * layout measured faster for the step with the means), keeping their values:
+ * the move briefly holds the moments twice (10 * n * element size more), and
+ * when that memory is not available the arrays stay where they are.
+ * LBM_ERR_NOMEM when an FO array cannot be allocated. */
 int lbm_fo_enable(lbm_handle *h, unsigned mask);
 /* Reads an FO field.  LBM_FIELD_K/KMEAN: n values; S/SMEAN: 6*n values in the
- * order xx,xy,xz,yy,yz,zz, component-major. */
+ * order xx,xy,xz,yy,yz,zz, component-major; MOMENTSMEAN: 10*n values, the means
+ * of the stored moments in the order of lbm_set_moments (rhoMean is the mean of
+ * rho-1, as the reference averages the stored arrays), component-major. */
 int lbm_fo_get(lbm_handle *h, int field, double *out, int layout);
 
 /* Global FP64 reductions (no reference counterpart): kinetic energy

@@ -32,8 +32,11 @@
  *   omega ................ latticeMesh.cuh:178-182 (pinned: closed-form values)
  *   initial fields ....... hostArray.cuh:304-359, boundaryValue.cuh:147-254
  *   function objects ..... kineticEnergy.cuh:71-85, strainRateTensor.cuh:71-84,
- *                          functionObjects.cuh:79-83 (pinned: closed forms,
- *                          running mean of 1..n = (n+1)/2)
+ *                          functionObjects.cuh:79-83, moments.cuh:70-101
+ *                          (pinned: closed forms, running mean of 1..n =
+ *                          (n+1)/2; lbmo_run_means against step-by-step
+ *                          lbmo_k/lbmo_S/lbmo_time_average and the mean of a
+ *                          stationary state)
  *   tile source map ...... streaming.cuh:126-154, halo.cuh:171-293 (load table),
  *                          halo.cuh:813-904 (save slots), globalFunctions.cuh:415-476
  *                          pinned: equals periodic(x - c_q) for every node, and
@@ -392,6 +395,47 @@ int lbmo_run(int Q, int nx, int ny, int nz, double omega, double u_lid, double *
     for (int s = 0; s < nsteps; s++) {
         all_pops(Q, n, (const double *const *)mom, F, nt);
         step_with(Q, nx, ny, nz, omega, u_lid, F, mom, nt);
+    }
+    free(F);
+    return 0;
+}
+
+/* n steps with the function objects' running means accumulated after every
+ * step, as the reference driver calls runTimeObjects.calculate(t) after the
+ * step of time t (momentBasedD3Q19.cu:126-136): mean += (x - mean)/(t + 1)
+ * with t = t0 + s for the s-th step of this call (timeAverage,
+ * functionObjects.cuh:79-83).  x is k (kineticEnergy.cuh:71-85), the six S
+ * components (strainRateTensor.cuh:71-84) and the ten stored moments
+ * (moments.cuh:70-101), all from the post-step stored moments.  Any mean
+ * pointer may be NULL. */
+int lbmo_run_means(int Q, int nx, int ny, int nz, double omega, double u_lid, double *const *mom, int nsteps,
+                   int nthreads, int64_t t0, double tau, double *kmean, double *smean, double *mmean) {
+    if (Q != 19 && Q != 27) return -1;
+    if (nx < 2 || ny < 2 || nz < 2) return -2;
+    const int nt = set_threads(nthreads);
+    const int64_t n = (int64_t)nx * ny * nz;
+    double *F = (double *)malloc(sizeof(double) * (size_t)n * Q);
+    if (!F) return -3;
+    const double sd = 3.0 / (2.0 * 4.5 * tau), so = 3.0 / (2.0 * 9.0 * tau);
+    for (int s = 0; s < nsteps; s++) {
+        all_pops(Q, n, (const double *const *)mom, F, nt);
+        step_with(Q, nx, ny, nz, omega, u_lid, F, mom, nt);
+        const double inv = 1.0 / (double)(t0 + s + 1);
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int64_t i = 0; i < n; i++) {
+            const double u = mom[1][i], v = mom[2][i], w = mom[3][i];
+            if (kmean) {
+                const double k = 0.5 * sqrt(u * u + v * v + w * w);
+                kmean[i] += (k - kmean[i]) * inv;
+            }
+            if (smean) {
+                const double S[6] = {(u * u - mom[4][i]) * sd, (u * v - mom[5][i]) * so, (u * w - mom[6][i]) * so,
+                                     (v * v - mom[7][i]) * sd, (v * w - mom[8][i]) * so, (w * w - mom[9][i]) * sd};
+                for (int a = 0; a < 6; a++) smean[a * n + i] += (S[a] - smean[a * n + i]) * inv;
+            }
+            if (mmean)
+                for (int a = 0; a < 10; a++) mmean[a * n + i] += (mom[a][i] - mmean[a * n + i]) * inv;
+        }
     }
     free(F);
     return 0;

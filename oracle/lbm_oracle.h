@@ -78,6 +78,14 @@ int lbmo_step(int Q, int nx, int ny, int nz, double omega, double u_lid,
 int lbmo_run(int Q, int nx, int ny, int nz, double omega, double u_lid,
              double *const *mom, int nsteps, int nthreads);
 
+/* lbmo_run with the function objects' FP64 running means updated after every
+ * step: kmean (n), smean (6*n, xx,xy,xz,yy,yz,zz) and mmean (10*n, the stored
+ * moments), each may be NULL; t0 = absolute index of the first step
+ * (mean += (x - mean)/(t0 + s + 1)); tau as in lbmo_S. */
+int lbmo_run_means(int Q, int nx, int ny, int nz, double omega, double u_lid,
+                   double *const *mom, int nsteps, int nthreads, int64_t t0, double tau,
+                   double *kmean, double *smean, double *mmean);
+
 /* Initial field for one moment from its 7 region values (already scaled as the
  * reference scales them): order North, South, East, West, Front, Back,
  * internalField (hostArray.cuh:304-359).  out is x-fastest global order. */

@@ -50,6 +50,8 @@ def lib():
         pp = ctypes.POINTER(_dp)
         L.lbmo_step.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, ctypes.c_double, pp, pp, ctypes.c_int]
         L.lbmo_run.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, ctypes.c_double, pp, ctypes.c_int, ctypes.c_int]
+        L.lbmo_run_means.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, ctypes.c_double, pp, ctypes.c_int,
+                                                         ctypes.c_int, ctypes.c_int64, ctypes.c_double, _dp, _dp, _dp]
         L.lbmo_initial_field.argtypes = [ctypes.c_int] * 3 + [_dp, _dp]
         L.lbmo_initial_field.restype = None
         L.lbmo_k.argtypes = [pp, ctypes.c_int64, _dp]
@@ -151,6 +153,21 @@ def run(Q, nx, ny, nz, omega_, u_lid, mom, nsteps, nthreads=0):
     rc = lib().lbmo_run(Q, nx, ny, nz, omega_, u_lid, _ptrs(mom), nsteps, nthreads)
     assert rc == 0, rc
     return mom
+
+
+def run_means(Q, nx, ny, nz, omega_, u_lid, mom, nsteps, tau, means=None, t0=0, nthreads=0):
+    """Advance `mom` by nsteps in place, updating the FP64 running means after
+    every step.  means: dict with any of "kMean" (n), "SMean" (6, n) and
+    "momentsMean" (10, n) float64 arrays, created as zeros when absent; returns it."""
+    n = nx * ny * nz
+    means = {} if means is None else means
+    for key, shape in (("kMean", (n,)), ("SMean", (6, n)), ("momentsMean", (10, n))):
+        means.setdefault(key, np.zeros(shape))
+        assert means[key].dtype == np.float64 and means[key].flags.c_contiguous
+    rc = lib().lbmo_run_means(Q, nx, ny, nz, omega_, u_lid, _ptrs(mom), nsteps, nthreads, t0, tau,
+                              _d(means["kMean"]), _d(means["SMean"]), _d(means["momentsMean"]))
+    assert rc == 0, rc
+    return means
 
 
 def step(Q, nx, ny, nz, omega_, u_lid, mom_in, nthreads=0):

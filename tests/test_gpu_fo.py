@@ -124,26 +124,85 @@ def test_layout_tiled8_round_trip(native_lib):
         assert np.array_equal(back_t[k], tiled[k])
 
 
-@pytest.mark.parametrize("Q", [19, 27])
-def test_fo_enable_keeps_the_moments(native_lib, Q):
-    """Enabling or disabling the FP64 means may move the moment arrays to
-    another layout in device memory; the moments themselves are unchanged."""
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_fo_enable_keeps_the_moments(native_lib, storage):
+    """Turning the FP64 means on moves the D3Q19 moment arrays into the shifted
+    layout and turning them off moves them back; the moments survive both moves
+    bit for bit, and a run with both moves equals a run with none (the step is
+    deterministic and the layout changes no arithmetic)."""
     from cudalbm_amd import Solver
     from cudalbm_amd import native as N
 
     n = 32
-    with Solver(n, n, n, 400.0, 0.05, velocity_set=Q) as s:
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=19, storage=storage) as s:
         s.step(30)
         a = s.get_moments()
-        s.fo_enable(N.LBM_FO_KMEAN | N.LBM_FO_SMEAN)
+        s.fo_enable(N.LBM_FO_KMEAN | N.LBM_FO_SMEAN | N.LBM_FO_MOMENTSMEAN)
         b = s.get_moments()
         s.step(5)
-        s.fo_enable(0)
         c = s.get_moments()
+        s.fo_enable(0)
+        c2 = s.get_moments()
         s.step(5)
         d = s.get_moments()
-    print(f"D3Q{Q}: moments kept across both layout changes", flush=True)
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=19, storage=storage) as s:
+        s.step(40)
+        e = s.get_moments()
+    print(f"D3Q19 {storage}: moments kept across both layout changes", flush=True)
     for k in range(10):
         assert np.array_equal(a[k], b[k]), k
-    assert all(np.isfinite(np.asarray(d[k])).all() for k in range(10))
+        assert np.array_equal(c[k], c2[k]), k
+        assert np.array_equal(d[k], e[k]), k
     assert not np.array_equal(c[1], d[1])
+
+
+def test_fo_enable_no_move_d3q27(native_lib):
+    """D3Q27 keeps one (shifted) layout whatever the function objects: enabling
+    the means leaves the moments untouched."""
+    from cudalbm_amd import Solver
+    from cudalbm_amd import native as N
+
+    n = 32
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=27) as s:
+        s.step(20)
+        a = s.get_moments()
+        s.fo_enable(N.LBM_FO_KMEAN | N.LBM_FO_MOMENTSMEAN)
+        b = s.get_moments()
+    for k in range(10):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_function_object_means_against_oracle(native_lib, Q):
+    """kMean, SMean and momentsMean accumulated in FP64 inside the GPU step
+    against the FP64 oracle's running means of its own trajectory
+    (lbmo_run_means), started at absolute step t0 = 100 so that the device step
+    clock's 1/(t+1) is exercised away from zero.  Tolerance: rel-L2 per field
+    <= 1e-5 (the step's parity bar) for the velocity means and kMean, <= 1e-4
+    for the small second-moment and density means (their values are O(U^2))."""
+    from cudalbm_amd import Solver
+    from cudalbm_amd import native as N
+
+    n, Re, u, steps, t0 = 32, 400.0, 0.05, 60, 100
+    om = O.omega(u, Re, n)
+    mom = O.rest_state(n, n, n)
+    ref = O.run_means(Q, n, n, n, om, u, mom, steps, 1.0 / om, t0=t0)
+    with Solver(n, n, n, Re, u, velocity_set=Q) as s:
+        s.time = t0
+        s.fo_enable(N.LBM_FO_KMEAN | N.LBM_FO_SMEAN | N.LBM_FO_MOMENTSMEAN)
+        s.step(steps)
+        km = s.fo_get(N.LBM_FIELD_KMEAN)
+        sm = s.fo_get(N.LBM_FIELD_SMEAN)
+        mm = s.fo_get(N.LBM_FIELD_MOMENTSMEAN)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    errs = {"kMean": rel(km, ref["kMean"])}
+    errs.update({f"SMean[{a}]": rel(sm[a], ref["SMean"][a]) for a in range(6)})
+    errs.update({f"momentsMean[{a}]": rel(mm[a], ref["momentsMean"][a]) for a in range(10)})
+    print(f"D3Q{Q} means vs oracle: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert errs["kMean"] <= 1e-5
+    for a in (1, 2, 3):
+        assert errs[f"momentsMean[{a}]"] <= 1e-5
+    assert max(errs.values()) <= 1e-4

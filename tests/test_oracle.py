@@ -448,3 +448,32 @@ def test_cavity_z_symmetry():
     assert np.abs(u - u[::-1]).max() < 1e-12
     assert np.abs(w + w[::-1]).max() < 1e-12
     assert np.abs(u).max() > 1e-3
+
+
+def test_run_means_follows_the_step_by_step_definition():
+    """lbmo_run_means equals stepping one step at a time and updating each mean
+    with the separately pinned k, S and time-average functions after every
+    step, starting at absolute step t0 (mean += (x - mean)/(t + 1)); a run from
+    the rest state with the lid at rest keeps every mean at its rest value."""
+    n, Q, Re, u, steps, t0 = 8, 19, 100.0, 0.05, 6, 3
+    om = O.omega(u, Re, n)
+    tau = 1.0 / om
+    mom_a = O.rest_state(n, n, n)
+    means = O.run_means(Q, n, n, n, om, u, mom_a, steps, tau, t0=t0, nthreads=1)
+    mom_b = O.rest_state(n, n, n)
+    km, sm, mm = np.zeros(n ** 3), np.zeros((6, n ** 3)), np.zeros((10, n ** 3))
+    for s in range(steps):
+        mom_b = O.step(Q, n, n, n, om, u, mom_b, nthreads=1)
+        km = O.time_average(km, O.k_field(mom_b), t0 + s)
+        S = O.S_field(mom_b, tau)
+        sm = np.stack([O.time_average(sm[a], S[a], t0 + s) for a in range(6)])
+        mm = np.stack([O.time_average(mm[a], mom_b[a], t0 + s) for a in range(10)])
+    for a, b in zip(mom_a, mom_b):
+        assert np.array_equal(a, b)
+    assert np.abs(means["kMean"] - km).max() < 1e-15
+    assert np.abs(means["SMean"] - sm).max() < 1e-15
+    assert np.abs(means["momentsMean"] - mm).max() < 1e-15
+    assert np.abs(means["kMean"]).max() > 1e-6  # the check is not vacuous
+    rest = O.run_means(Q, n, n, n, om, 0.0, O.rest_state(n, n, n), 4, tau, nthreads=1)
+    for key in ("kMean", "SMean", "momentsMean"):
+        assert np.abs(rest[key]).max() < 1e-15
