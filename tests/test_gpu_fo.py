@@ -178,8 +178,8 @@ def test_function_object_means_against_oracle(native_lib, Q):
     against the FP64 oracle's running means of its own trajectory
     (lbmo_run_means), started at absolute step t0 = 100 so that the device step
     clock's 1/(t+1) is exercised away from zero.  Tolerance: rel-L2 per field
-    <= 1e-5 (the step's parity bar) for the velocity means and kMean, <= 1e-4
-    for the small second-moment and density means (their values are O(U^2))."""
+    <= 1e-5, the step's parity bar (measured on MI355X: at most 6e-7 for D3Q19
+    and 1.5e-6 for D3Q27)."""
     from cudalbm_amd import Solver
     from cudalbm_amd import native as N
 
@@ -202,7 +202,4 @@ def test_function_object_means_against_oracle(native_lib, Q):
     errs.update({f"SMean[{a}]": rel(sm[a], ref["SMean"][a]) for a in range(6)})
     errs.update({f"momentsMean[{a}]": rel(mm[a], ref["momentsMean"][a]) for a in range(10)})
     print(f"D3Q{Q} means vs oracle: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
-    assert errs["kMean"] <= 1e-5
-    for a in (1, 2, 3):
-        assert errs[f"momentsMean[{a}]"] <= 1e-5
-    assert max(errs.values()) <= 1e-4
+    assert max(errs.values()) <= 1e-5, errs
