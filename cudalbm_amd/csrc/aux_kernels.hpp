@@ -119,4 +119,13 @@ __global__ __launch_bounds__(NT) void residual_final_kernel(const double *__rest
     }
 }
 
+// Kinetic energy and residual r = sqrt(sum |u - u_prev|^2 / sum |u|^2) from the
+// three sums of residual_final_kernel, on the device (one thread).
+static __global__ void residual_result_kernel(const double *__restrict__ sums, double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double du = sums[1], uu = sums[2];
+    out[0] = sums[0];
+    out[1] = uu > 0 ? sqrt(du / uu) : (du > 0 ? INFINITY : 0.0);
+}
+
 }  // namespace lbm

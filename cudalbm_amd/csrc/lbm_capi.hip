@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -21,6 +22,32 @@ using namespace lbm;
 namespace {
 
 thread_local std::string g_create_error;
+
+// ---------------------------------------------- scratch of lbm_reduce_noalloc
+// The per-block FP64 partial sums of the residual reduction (3 per block) and
+// the three totals, in device memory that belongs to the library's code object
+// rather than to any allocation: one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself (in host code the name is a stand-in,
+// not a device address); it uses the address hipGetSymbolAddress returns for
+// the current device, looked up once per device by scratch_on().
+constexpr int kScratchBlocks = 1024;  // reduction blocks whose partials fit
+constexpr int kScratchWords = 3 * kScratchBlocks + 3;
+static __device__ __align__(256) double scratch[kScratchWords];
+
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;
+
+// Device address of `scratch` on `device` (which must be current).
+double *scratch_on(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) return it->second;
+    double *ptr = nullptr;
+    HIPX(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw HipError{hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)"};
+    g_scratch_ptr[device] = ptr;
+    return ptr;
+}
 
 // ------------------------------------------------------------ dispatch
 template <int Q, int TX, int TY>
@@ -233,7 +260,8 @@ struct lbm_handle {
     int64_t t = 0;
     float *uprev = nullptr;
     double *part = nullptr, *red_out = nullptr;
-    int red_blocks = 1024;
+    int red_blocks = 1024;  // equal to kScratchBlocks: lbm_reduce_noalloc sums in the same order
+    double *scratch = nullptr; // this device's copy of `scratch` (lbm_reduce_noalloc)
     bool red_first = true;
     std::map<int, hipGraphExec_t> graphs;  // key: parity*2 + means flag, for kGraphSteps steps
     std::map<int, std::vector<int64_t>> perms;
@@ -577,6 +605,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
             HIPX(hipMemsetAsync(h->faces[s], 0, sizeof(float) * h->fwords, h->stream));
         }
         dev_alloc((void **)&h->clock, 2 * sizeof(StepClock), "step clock");
+        h->scratch = scratch_on(c.device);
         set_clock(h);
         rebuild_faces(h);
         HIPX(hipStreamSynchronize(h->stream));
@@ -821,6 +850,34 @@ int lbm_reduce(lbm_handle *h, double *ke, double *res) {
         h->red_first = false;
         if (ke) *ke = r[0];
         if (res) *res = r[2] > 0 ? std::sqrt(r[1] / r[2]) : (r[1] > 0 ? INFINITY : 0.0);
+    });
+}
+
+int lbm_reduce_noalloc(lbm_handle *h, float *uprev, int64_t uprev_count, int first, double *out) {
+    return guarded(h, [&] {
+        if (!uprev || !out) throw ArgError{LBM_ERR_ARG, "null device buffer"};
+        if (uprev_count < 3 * h->n)
+            throw ArgError{LBM_ERR_SIZE, "previous-velocity buffer holds " + std::to_string(uprev_count) +
+                                             " floats; this mesh needs 3 * nodes = " + std::to_string(3 * h->n)};
+        if (h->red_blocks > kScratchBlocks)
+            throw ArgError{LBM_ERR_SIZE, "reduction needs " + std::to_string(h->red_blocks) +
+                                             " partial blocks; the static scratch holds " +
+                                             std::to_string(kScratchBlocks)};
+        if (!h->scratch) throw ArgError{LBM_ERR_STATE, "static scratch not available on this device"};
+        double *part = h->scratch, *sums = h->scratch + 3 * kScratchBlocks;
+        const StepParams p = h->params();
+        constexpr int NT = 256;
+        if (h->cfg.storage == LBM_STORE_FP32)
+            hipLaunchKernelGGL((residual_partials_kernel<float, NT>), dim3(h->red_blocks), dim3(NT), 0, h->stream, p,
+                               uprev, part, first ? 1 : 0);
+        else
+            hipLaunchKernelGGL((residual_partials_kernel<_Float16, NT>), dim3(h->red_blocks), dim3(NT), 0,
+                               h->stream, p, uprev, part, first ? 1 : 0);
+        HIPX(hipGetLastError());
+        hipLaunchKernelGGL((residual_final_kernel<NT>), dim3(1), dim3(NT), 0, h->stream, part, h->red_blocks, sums);
+        HIPX(hipGetLastError());
+        hipLaunchKernelGGL(residual_result_kernel, dim3(1), dim3(64), 0, h->stream, sums, out);
+        HIPX(hipGetLastError());
     });
 }
 

@@ -25,7 +25,7 @@ EXPORTED = [
     "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_create_error", "lbm_get_tiling", "lbm_get_omega",
     "lbm_set_moments", "lbm_get_moments", "lbm_set_moments_f64", "lbm_get_moments_f64", "lbm_step",
     "lbm_synchronize", "lbm_step_timed", "lbm_get_time", "lbm_set_time", "lbm_fo_enable", "lbm_fo_get",
-    "lbm_reduce", "lbm_probe_streaming",
+    "lbm_reduce", "lbm_reduce_noalloc", "lbm_probe_streaming",
 ]
 
 
@@ -84,6 +84,7 @@ def lib():
     L.lbm_fo_enable.argtypes = [H, ctypes.c_uint]
     L.lbm_fo_get.argtypes = [H, ctypes.c_int, dp, ctypes.c_int]
     L.lbm_reduce.argtypes = [H, dp, dp]
+    L.lbm_reduce_noalloc.argtypes = [H, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
     L.lbm_probe_streaming.argtypes = [H, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8)]
     _lib = L
     return L
@@ -207,6 +208,13 @@ class Solver:
         ke, r = ctypes.c_double(), ctypes.c_double()
         self._check(lib().lbm_reduce(self._h, ctypes.byref(ke), ctypes.byref(r)))
         return ke.value, r.value
+
+    def reduce_noalloc(self, uprev_ptr, uprev_count, first, out_ptr):
+        """lbm_reduce_noalloc: device pointers (ints) to the caller's previous-
+        velocity buffer (uprev_count floats) and to 2 doubles of output;
+        asynchronous on the solver's stream."""
+        self._check(lib().lbm_reduce_noalloc(self._h, ctypes.c_void_p(uprev_ptr), int(uprev_count), int(first),
+                                             ctypes.c_void_p(out_ptr)))
 
     def probe_streaming(self):
         src = np.empty(self.n * self.Q, dtype=np.uint32)

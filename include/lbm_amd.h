@@ -143,6 +143,25 @@ int lbm_fo_get(lbm_handle *h, int field, double *out, int layout);
  * previous call (r = 1 on the first call).  Deterministic (fixed-order). */
 int lbm_reduce(lbm_handle *h, double *kinetic_energy, double *residual);
 
+/* No counterpart in the reference.  lbm_reduce for callers that cannot
+ * allocate device memory or synchronize: code under HIP graph capture and
+ * tight solve loops.  It allocates nothing (no hipMalloc/hipFree, no
+ * allocating library call) and only enqueues kernels on the handle's stream.
+ *   uprev        device buffer of uprev_count floats, owned by the caller,
+ *                at least 3 * nodes: the previous velocity field in the
+ *                library's internal order; read (unless `first`) and then
+ *                overwritten with the current field.
+ *   first        nonzero: no previous field yet; the residual is reported as 1
+ *                (as on lbm_reduce's first call).
+ *   out          device memory for 2 doubles: kinetic energy, residual.
+ * The per-block partial sums live in a fixed static device buffer of the
+ * library (1024 blocks x 3 doubles + 3, 24 KiB per GPU, kept until the
+ * process ends), shared by every handle on that GPU: calls that overlap on
+ * one GPU (two streams, or two threads) race and must be serialized by the
+ * caller.  LBM_ERR_SIZE if uprev_count < 3 * nodes.  Results equal
+ * lbm_reduce's bit for bit (same blocks, same summation order). */
+int lbm_reduce_noalloc(lbm_handle *h, float *uprev, int64_t uprev_count, int first, double *out);
+
 /* Integer probes of the streaming/halo machinery (test hooks; no reference
  * counterpart).  src_out[d*Q+q] receives 1 + 32*s + q where s is the global
  * x-fastest index of the node whose population q arrives at node d, or 0 when
