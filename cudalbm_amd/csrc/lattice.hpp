@@ -19,6 +19,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Floating-point contraction is off for everything in this file: a multiply
+// and an add fuse into an FMA only where the source says fmaf, so every kernel
+// that inlines these functions (the step of any tiling, the batched step, the
+// face rebuild) rounds exactly alike and a restart or another tiling
+// reproduces the same bits.
+#pragma float_control(push)
+#pragma clang fp contract(off)
 namespace lbm {
 
 constexpr int kC[27][3] = {
@@ -359,12 +366,12 @@ template <int Q>
 __device__ __forceinline__ void boundary_moments(const Sums &s, const Closure &k, float (&m)[10]) {
     const float rI = div_d3<Q>(s.s0);
     const float a[3] = {div_d3<Q>(s.sxy), div_d3<Q>(s.sxz), div_d3<Q>(s.syz)};
-    const float rp = (k.A * rI + k.Bxy * a[0] + k.Bxz * a[1] + k.Byz * a[2] + k.C) * k.invD;
+    const float rp = fmaf(k.A, rI, fmaf(k.Bxy, a[0], fmaf(k.Bxz, a[1], fmaf(k.Byz, a[2], k.C)))) * k.invD;
     const float rho = 1.f + rp;
     const float ir = recip(rho);
     float mab[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) mab[j] = (k.E[j] * a[j] + k.F[j] * rp + k.G[j] * rho) * (k.invH[j] * ir);
+    for (int j = 0; j < 3; j++) mab[j] = fmaf(k.E[j], a[j], fmaf(k.F[j], rp, k.G[j] * rho)) * (k.invH[j] * ir);
     m[0] = rp;
     m[1] = 3.f * k.ux;
     m[2] = 0.f;
@@ -395,3 +402,4 @@ __host__ __device__ __forceinline__ int node_bits(int x, int y, int z, int nx, i
 }
 
 }  // namespace lbm
+#pragma float_control(pop)

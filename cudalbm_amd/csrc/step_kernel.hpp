@@ -26,6 +26,13 @@
 
 #include "lattice.hpp"
 
+// Floating-point contraction is off for everything in this file: a multiply
+// and an add fuse into an FMA only where the source says fmaf, so every kernel
+// that inlines these functions (the step of any tiling, the batched step, the
+// face rebuild) rounds exactly alike and a restart or another tiling
+// reproduces the same bits.
+#pragma float_control(push)
+#pragma clang fp contract(off)
 namespace lbm {
 
 // Barrier that orders LDS traffic only: global loads issued for the next plane
@@ -305,22 +312,18 @@ __device__ __forceinline__ void write_y_faces(const StepParams &p, float *__rest
         });
 }
 template <int Q, int TX, int TY, bool PROBE>
-__device__ __forceinline__ void write_xy_faces(const StepParams &p, float *__restrict__ fout,
-                                               const float (&m)[10], int tx, int ty, int x, int y,
-                                               int z, int bx, int by) {
-    const Rec r = rec_setup(m);
+__device__ __forceinline__ void write_xy_faces(const StepParams &p, float *__restrict__ fout, const Rec &r, int tx,
+                                               int ty, int x, int y, int z, int bx, int by) {
     write_x_faces<Q, TX, TY, PROBE>(p, fout, r, tx, x, y, z, bx);
     write_y_faces<Q, TX, TY, PROBE>(p, fout, r, ty, x, y, z, by);
 }
 
-// Outgoing z face populations at the chunk's first and last planes.
+// Outgoing z face populations of a node on the chunk's first plane (down-going,
+// read by chunk c-1) or last plane (up-going, read by chunk c+1).
 template <int Q, bool PROBE>
-__device__ __forceinline__ void write_z_faces(const StepParams &p, float *__restrict__ fout,
-                                              const float (&m)[10], int x, int y, int z, int c, int z0,
-                                              int z1) {
+__device__ __forceinline__ void write_z_faces(const StepParams &p, float *__restrict__ fout, const Rec &r, int x,
+                                              int y, int z, int c, int z0, int z1) {
     constexpr int QF = VS<Q>::QF;
-    if (!((z == z0 && c > 0) || (z == z1 && c < p.nc - 1))) return;
-    const Rec r = rec_setup(m);
     if (z == z0 && c > 0)
         unroll<0, QF>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
@@ -333,14 +336,6 @@ __device__ __forceinline__ void write_z_faces(const StepParams &p, float *__rest
             constexpr int q = face_q<Q>(2, 1, k);
             fout[zf_off<Q>(p, 1, c, k, y, x)] = pop_of<Q, q, PROBE>(r, x, y, z, p);
         });
-}
-
-template <int Q, int TX, int TY, bool PROBE>
-__device__ __forceinline__ void write_faces(const StepParams &p, float *__restrict__ fout,
-                                            const float (&m)[10], int tx, int ty, int x, int y,
-                                            int z, int bx, int by, int c, int z0, int z1) {
-    write_xy_faces<Q, TX, TY, PROBE>(p, fout, m, tx, ty, x, y, z, bx, by);
-    write_z_faces<Q, PROBE>(p, fout, m, x, y, z, c, z0, z1);
 }
 
 // Edge nodes of the footprint (the ones that own x/y face populations),
@@ -424,14 +419,6 @@ __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__res
 template <int TX, int TY>
 __device__ __forceinline__ unsigned plane_base(const StepParams &p, int bx, int by, int z) {
     return (unsigned)(((z * p.nby + by) * p.nbx + bx) * (TX * TY));
-}
-
-template <typename ST, int TX, int TY>
-__device__ __forceinline__ void load_plane(const StepParams &p, int bx, int by, int z, int tid,
-                                           float (&m)[10]) {
-    const unsigned off = (plane_base<TX, TY>(p, bx, by, z) + (unsigned)tid) * Store<ST>::kBytes;
-#pragma unroll
-    for (int k = 0; k < 10; k++) m[k] = Store<ST>::load(p.mom[k], off, codec_dec(k));
 }
 
 // The same plane as stored (FP16: not yet scaled by the codec's decode
@@ -570,7 +557,9 @@ __global__ __launch_bounds__(TX *TY, PERSIST ? (Q == 19 ? 3 : 2) : (Q == 19 ? 4 
 
 // Face buffers from the current moments (post-collision populations of every
 // face node): the device-side replacement of haloFace::initialise_pop
-// (haloFace.cuh:244-449).  Same grid as the step.
+// (haloFace.cuh:244-449).  Same grid as the step, and the same rebuild from the
+// moments as stored (rec_setup_st) as the step's own face writes, so a restart
+// reproduces the faces an uninterrupted run would hold bit for bit.
 template <int Q, int TX, int TY, typename ST, bool PROBE>
 __global__ __launch_bounds__(TX *TY) void fill_faces_kernel(const StepParams p) {
     const int tid = threadIdx.x;
@@ -584,11 +573,14 @@ __global__ __launch_bounds__(TX *TY) void fill_faces_kernel(const StepParams p) 
     for (int z = z0; z <= z1; z++) {
         float m[10];
         if constexpr (!PROBE)
-            load_plane<ST, TX, TY>(p, bx, by, z, tid, m);
+            load_plane_raw<ST, TX, TY>(p, bx, by, z, tid, m);
         else
             for (int k = 0; k < 10; k++) m[k] = 0.f;
-        write_faces<Q, TX, TY, PROBE>(p, p.fout, m, tx, ty, x, y, z, bx, by, c, z0, z1);
+        const Rec r = rec_setup_st<ST>(m);
+        write_xy_faces<Q, TX, TY, PROBE>(p, p.fout, r, tx, ty, x, y, z, bx, by);
+        write_z_faces<Q, PROBE>(p, p.fout, r, x, y, z, c, z0, z1);
     }
 }
 
 }  // namespace lbm
+#pragma float_control(pop)

@@ -1,14 +1,11 @@
 """GPU tests of the batched multi-case sweep (lbm_batch_*) and the autotuners.
 
-The batch kernel runs the same step body as the single-case kernel but is a
-separate compilation, and the compiler contracts a few different multiply-adds
-into FMAs (220 vs 215 v_fma_f32 at 16x16), so each case matches a single-case
-run of that case to FP32 rounding, not bitwise (observed: 1-ulp differences
-of the largest values).  Tolerance: for every moment, max |difference| <=
-1e-6 x the largest stored-moment magnitude of the case, and velocity rel-L2 <=
-1e-6.  A wrong parameter or a stale face buffer moves fields by >= 1e-4 of
-that scale.  Residual/energy reductions: relative 1e-6.  One case is also
-checked directly against the FP64 oracle (velocity rel-L2 <= 1e-5).
+The batch kernel runs the same step body as the single-case kernel, with the
+arithmetic fixed by the source (no implicit FMA contraction), so each case
+equals a single-case run of that case bit for bit.  Residual/energy
+reductions: relative 1e-12 (FP64 sums over different block partitions).  One
+case is also checked directly against the FP64 oracle (velocity rel-L2 <=
+1e-5).
 """
 import numpy as np
 import pytest
@@ -37,16 +34,10 @@ def single(case, steps, tile):
 
 
 def assert_same(a, b, what):
-    x = [m.astype(np.float64) for m in a]
-    y = [m.astype(np.float64) for m in b]
-    scale = max(np.abs(m).max() for m in y)
     for k in range(10):
-        d = np.abs(x[k] - y[k]).max()
-        if d > 1e-6 * scale:
-            raise AssertionError(f"{what}: moment {k} differs by {d:.3e} (scale {scale:.3e})")
-    vx, vy = np.stack(x[1:4]), np.stack(y[1:4])
-    rel = np.linalg.norm(vx - vy) / max(np.linalg.norm(vy), 1e-30)
-    assert rel <= 1e-6, f"{what}: velocity rel-L2 {rel:.3e}"
+        if not np.array_equal(a[k], b[k]):
+            d = float(np.abs(a[k].astype(np.float64) - b[k]).max())
+            raise AssertionError(f"{what}: moment {k} differs by up to {d:.3e}")
 
 
 @pytest.mark.parametrize("persistent", [0, 1])
@@ -123,9 +114,10 @@ def test_batch_reduce_matches_single(native_lib):
             k1, q1 = s.reduce()
             s.step(10)
             k2, q2 = s.reduce()
-        assert abs(ke1[i] - k1) <= 1e-6 * k1
-        assert abs(ke2[i] - k2) <= 1e-6 * k2
-        assert abs(r2[i] - q2) <= 1e-6 * q2
+        # equal fields; the FP64 sums may run over another block partition
+        assert abs(ke1[i] - k1) <= 1e-12 * k1
+        assert abs(ke2[i] - k2) <= 1e-12 * k2
+        assert abs(r2[i] - q2) <= 1e-12 * q2
 
 
 def test_batch_run_stops_each_case(native_lib):

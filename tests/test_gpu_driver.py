@@ -93,8 +93,10 @@ def test_driver_restart(tmp_path, native_lib):
     fb = fields(os.path.join(b, "lidDrivenCavity_60.LBMBin"))
     diffs = [float(np.abs(fa[k] - fb[k]).max()) for k in range(10)]
     print(f"restart vs uninterrupted, max |diff| per moment: {diffs}", flush=True)
+    # the face buffers are rebuilt from the saved moments with the same
+    # arithmetic the step uses to write them: the restart is bitwise
     for k in range(10):
-        assert diffs[k] <= 1e-6, k
+        assert np.array_equal(fa[k], fb[k]), (k, diffs[k])
 
 
 def test_driver_function_objects(tmp_path, native_lib):
