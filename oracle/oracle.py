@@ -155,17 +155,25 @@ def run(Q, nx, ny, nz, omega_, u_lid, mom, nsteps, nthreads=0):
     return mom
 
 
-def run_means(Q, nx, ny, nz, omega_, u_lid, mom, nsteps, tau, means=None, t0=0, nthreads=0):
+def run_means(Q, nx, ny, nz, omega_, u_lid, mom, nsteps, tau, means=None, t0=0, nthreads=0,
+              fields=("kMean", "SMean", "momentsMean")):
     """Advance `mom` by nsteps in place, updating the FP64 running means after
     every step.  means: dict with any of "kMean" (n), "SMean" (6, n) and
-    "momentsMean" (10, n) float64 arrays, created as zeros when absent; returns it."""
+    "momentsMean" (10, n) float64 arrays; the `fields` not in it are created as
+    zeros, the others are not computed.  Returns the dict."""
     n = nx * ny * nz
     means = {} if means is None else means
+    ptr = {}
     for key, shape in (("kMean", (n,)), ("SMean", (6, n)), ("momentsMean", (10, n))):
-        means.setdefault(key, np.zeros(shape))
-        assert means[key].dtype == np.float64 and means[key].flags.c_contiguous
+        if key in fields:
+            means.setdefault(key, np.zeros(shape))
+        if key in means:
+            assert means[key].dtype == np.float64 and means[key].flags.c_contiguous and means[key].shape == shape
+            ptr[key] = _d(means[key])
+        else:
+            ptr[key] = _dp()
     rc = lib().lbmo_run_means(Q, nx, ny, nz, omega_, u_lid, _ptrs(mom), nsteps, nthreads, t0, tau,
-                              _d(means["kMean"]), _d(means["SMean"]), _d(means["momentsMean"]))
+                              ptr["kMean"], ptr["SMean"], ptr["momentsMean"])
     assert rc == 0, rc
     return means
 
