@@ -176,7 +176,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="only the headline D3Q19 FP32 step")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-vs-cold leg")
     ap.add_argument("--only", default="",
-                    help="profile one variant: q19fo|q19fp32|q19fp16|q27fp32|q27fp16|batch|warm")
+                    help="profile one variant: q19fo|q19fp32|q19fp16|q19fp16fo|q27fp32|q27fp16|batch|warm")
     ap.add_argument("--tile", default="0,0,0")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (x4), to calibrate HBM byte counters")
@@ -206,6 +206,7 @@ def main():
             print(json.dumps({"variant": "warm", **warm_vs_cold()}), flush=True)
             return
         spec = {"q19fo": (19, "fp32", FO_ALL), "q19fp32": (19, "fp32", 0), "q19fp16": (19, "fp16", 0),
+                "q19fp16fo": (19, "fp16", FO_ALL),
                 "q27fp32": (27, "fp32", 0), "q27fp16": (27, "fp16", 0)}[args.only]
         r = measure(args.n, spec[0], spec[1], args.steps, args.warmup, fo_mask=spec[2], tile=tile)
         print(json.dumps({"variant": args.only, **r}), flush=True)

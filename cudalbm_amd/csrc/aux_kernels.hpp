@@ -119,6 +119,31 @@ __global__ __launch_bounds__(NT) void residual_final_kernel(const double *__rest
     }
 }
 
+// Column-tiled node order of one footprint: node (x, y, z) at
+// ((z nby + by) nbx + bx) tx ty + (y mod ty) tx + (x mod tx).
+struct ColumnTiling {
+    int tx, ty, nbx, nby;
+    __device__ __forceinline__ int64_t index(int x, int y, int z) const {
+        const int bx = x / tx, by = y / ty;
+        return (((int64_t)z * nby + by) * nbx + bx) * (int64_t)(tx * ty) + (int64_t)(y % ty) * tx + (x % tx);
+    }
+};
+
+// One array of n elements from the node order `from` into the order `to`
+// (a permutation: dst[to(x)] = src[from(x)]); grid-stride over the destination.
+template <typename T>
+__global__ void retile_kernel(const T *__restrict__ src, T *__restrict__ dst, int64_t n, ColumnTiling from,
+                              ColumnTiling to) {
+    const int64_t per = (int64_t)to.tx * to.ty, cols = (int64_t)to.nbx * to.nby;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t tile = i / per, r = i % per;
+        const int z = (int)(tile / cols), c = (int)(tile % cols);
+        const int by = c / to.nbx, bx = c % to.nbx;
+        const int y = by * to.ty + (int)(r / to.tx), x = bx * to.tx + (int)(r % to.tx);
+        dst[i] = src[from.index(x, y, z)];
+    }
+}
+
 // Kinetic energy and residual r = sqrt(sum |u - u_prev|^2 / sum |u|^2) from the
 // three sums of residual_final_kernel, on the device (one thread).
 static __global__ void residual_result_kernel(const double *__restrict__ sums, double *__restrict__ out) {

@@ -97,7 +97,7 @@ struct OccF {
 };
 
 bool batch_supported(int Q, int tx, int ty, bool persist) {
-    if (Q == 19) return tile_supported(tx, ty);
+    if (Q == 19) return (tx == 16 && ty == 16) || (tx == 32 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 16);
     return (tx == 16 && ty == 16) || (tx == 8 && ty == 8 && !persist);
 }
 

@@ -86,7 +86,12 @@ bool launch_step_q(int tx, int ty, const StepParams &p, int grid, hipStream_t s,
     else if (tx == 32 && ty == 8) launch_step_t<Q, 32, 8>(p, grid, s, storage, fo, probe);
     else if (tx == 8 && ty == 8) launch_step_t<Q, 8, 8>(p, grid, s, storage, fo, probe);
     else if (tx == 32 && ty == 16) launch_step_t<Q, 32, 16>(p, grid, s, storage, fo, probe);
-    else return false;
+    else if constexpr (Q == 19) {
+        // 1024-thread footprints (one workgroup per CU): D3Q19 only
+        if (tx == 32 && ty == 32) launch_step_t<Q, 32, 32>(p, grid, s, storage, fo, probe);
+        else if (tx == 64 && ty == 16) launch_step_t<Q, 64, 16>(p, grid, s, storage, fo, probe);
+        else return false;
+    } else return false;
     return true;
 }
 template <int Q>
@@ -95,7 +100,11 @@ bool launch_fill_q(int tx, int ty, const StepParams &p, int grid, hipStream_t s,
     else if (tx == 32 && ty == 8) launch_fill_t<Q, 32, 8>(p, grid, s, storage, probe);
     else if (tx == 8 && ty == 8) launch_fill_t<Q, 8, 8>(p, grid, s, storage, probe);
     else if (tx == 32 && ty == 16) launch_fill_t<Q, 32, 16>(p, grid, s, storage, probe);
-    else return false;
+    else if constexpr (Q == 19) {
+        if (tx == 32 && ty == 32) launch_fill_t<Q, 32, 32>(p, grid, s, storage, probe);
+        else if (tx == 64 && ty == 16) launch_fill_t<Q, 64, 16>(p, grid, s, storage, probe);
+        else return false;
+    } else return false;
     return true;
 }
 
@@ -103,8 +112,9 @@ bool launch_fill_q(int tx, int ty, const StepParams &p, int grid, hipStream_t s,
 
 namespace lbm {
 
-bool tile_supported(int tx, int ty) {
-    return (tx == 16 && ty == 16) || (tx == 32 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 16);
+bool tile_supported(int Q, int tx, int ty) {
+    return (tx == 16 && ty == 16) || (tx == 32 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 16) ||
+           (Q == 19 && ((tx == 32 && ty == 32) || (tx == 64 && ty == 16)));
 }
 
 void launch_step(int Q, int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool fo,
@@ -147,10 +157,13 @@ int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_
 // at 128^3 lz 16 (31.4k; 8: 27.7k), 192^3 lz 12 (40.0k; 16: 37.8k), 256^3 lz
 // 32 (42.6k; 64: 38.4k), 384^3 lz 48 (45.1k; 64: 44.0k); D3Q27 FP16 32x8 at
 // 256^3 lz 32 (49.1k; 16: 46.6k, 64: 42.1k), 384^3 lz 48 (53.7k; 32: 52.7k).
-int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool fp16) {
+// With the FP64 means on (mean_bytes per node read + written), a single round
+// pays no penalty either: D3Q19 32x32 at 256^3 lz 64 (616 us; 32: 636, 16: 649).
+int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool fp16, int mean_bytes) {
     const int64_t cols = (int64_t)(nx / tx) * (ny / ty);
     const double face = 16.0 * (Q == 19 ? 5 : 9);
-    const double mom = fp16 ? 40.0 : 80.0;
+    const double mom = (fp16 ? 40.0 : 80.0) + mean_bytes;
+    const bool single_round_penalty = !fp16 && mean_bytes == 0;
     int best = -1;
     double best_cost = 0.0;
     for (int lz = 4; lz <= std::min(nz, 64); lz++) {
@@ -158,7 +171,7 @@ int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool
         const int64_t items = cols * (nz / lz);
         const int64_t rounds = (items + slots - 1) / slots;
         const double bpn = mom + face * (1.0 / tx + 1.0 / ty + 1.0 / lz);
-        const double cost = (double)rounds * (lz + 4) * bpn * (rounds == 1 && !fp16 ? 1.1 : 1.0);
+        const double cost = (double)rounds * (lz + 4) * bpn * (rounds == 1 && single_round_penalty ? 1.1 : 1.0);
         if (best < 0 || cost < best_cost) {
             best = lz;
             best_cost = cost;
@@ -181,17 +194,36 @@ int64_t resident_workgroups(int Q, int tx, int ty, int device) {
     return (int64_t)cus * per_cu;
 }
 
-void choose_tile(int Q, int nx, int ny, bool fp16, int *tx, int *ty) {
+void choose_tile(int Q, int nx, int ny, bool fp16, bool means, int *tx, int *ty) {
     // measured on MI355X at 256^3: D3Q19 is fastest with 32x16 footprints (8 waves,
     // fewest face words per node); D3Q27's LDS allows 16x16 at 3 workgroups/CU.
+    // With the FP64 means on, the D3Q19 step runs 7% faster (FP32: 616 vs 661 us,
+    // FP16: 554 vs 568 us) on 32x32 footprints, one 16-wave workgroup per CU
+    // (without the means 32x32 is 4% slower: 272 vs 262 us).
     // D3Q27 with FP16 storage runs 9% faster at 32x8 (256^3: 342 vs 373 us;
     // 384^3: 1054 vs 1152 us); with FP32 storage 16x16 stays ahead at 384^3
     // (1234 vs 1294 us; 256^3: 396 vs 390 us).
-    if (Q == 19 && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
+    if (Q == 19 && means && nx % 32 == 0 && ny % 32 == 0) { *tx = 32; *ty = 32; }
+    else if (Q == 19 && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
     else if (Q == 27 && fp16 && nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
     else if (nx % 16 == 0 && ny % 16 == 0) { *tx = 16; *ty = 16; }
     else if (nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
     else { *tx = 8; *ty = 8; }
+}
+
+// Bytes per node of the FP64 running means a function-object mask turns on.
+int mean_bytes(unsigned fo_mask) {
+    return ((fo_mask & LBM_FO_KMEAN) ? 16 : 0) + ((fo_mask & LBM_FO_SMEAN) ? 96 : 0) +
+           ((fo_mask & LBM_FO_MOMENTSMEAN) ? 160 : 0);
+}
+
+// Automatic tiling of a case with a function-object mask.
+void auto_tiling(const lbm_config &c, unsigned fo_mask, int *tx, int *ty, int *lz) {
+    const bool fp16 = c.storage == LBM_STORE_FP16;
+    const int mb = mean_bytes(fo_mask);
+    choose_tile(c.velocity_set, c.nx, c.ny, fp16, mb > 0, tx, ty);
+    *lz = choose_lz(c.velocity_set, c.nx, c.ny, c.nz, *tx, *ty,
+                    resident_workgroups(c.velocity_set, *tx, *ty, c.device), fp16, mb);
 }
 
 // Internal (column-tiled) index of a node.
@@ -243,6 +275,7 @@ static void dev_alloc(void **p, size_t bytes, const char *what) {
 struct lbm_handle {
     lbm_config cfg{};
     int Q = 19, TX = 16, TY = 16, LZ = 16, nbx = 0, nby = 0, nc = 0;
+    bool auto_tile = false;  // tiling chosen by the library (may change with the function objects)
     int64_t n = 0;
     double omega = 0, tau = 0;
     hipStream_t stream = nullptr;
@@ -545,6 +578,91 @@ static void layout_moments(lbm_handle *h, unsigned fo_mask, bool keep) {
     for (int i = 0; i < 10; i++) h->mom[i] = static_cast<char *>(base) + stride * i;
 }
 
+// Moves the case to another automatic tiling (footprint tx x ty, lz planes):
+// the moments (and the residual's previous velocity field) are permuted on the
+// device into the new column-tiled order, the face buffers are reallocated for
+// the new geometry and rebuilt from the moments.  Everything new is allocated
+// before anything old is freed; if that memory is not available the case keeps
+// its tiling (a speed choice only) and false is returned.
+static bool retile(lbm_handle *h, int tx, int ty, int lz, unsigned fo_mask) {
+    const lbm_config &c = h->cfg;
+    int64_t yoff = 0, zoff = 0;
+    const int64_t fwords = face_words(h->Q, c.nx, c.ny, c.nz, tx, ty, lz, &yoff, &zoff);
+    if (fwords >= (int64_t)INT32_MAX) return false;
+    const bool shift = h->Q == 27 || (fo_mask & kMeanBits) != 0;
+    const size_t stride = (h->elem() * (size_t)h->n + 255) / 256 * 256 + (shift ? 4096 : 0);
+    void *base = nullptr;
+    float *faces[2] = {nullptr, nullptr}, *uprev = nullptr;
+    auto free_new = [&] {
+        for (void *p : {base, (void *)faces[0], (void *)faces[1], (void *)uprev})
+            if (p) (void)hipFree(p);
+    };
+    auto get = [&](void **p, size_t bytes) {
+        const hipError_t e = hipMalloc(p, bytes);
+        if (e == hipSuccess) return true;
+        (void)hipGetLastError();
+        *p = nullptr;
+        if (e != hipErrorOutOfMemory) {
+            free_new();
+            throw HipError{e, "hipMalloc (retile)"};
+        }
+        return false;
+    };
+    if (!get(&base, stride * 10) || !get((void **)&faces[0], sizeof(float) * fwords) ||
+        !get((void **)&faces[1], sizeof(float) * fwords) ||
+        (h->uprev && !get((void **)&uprev, sizeof(float) * 3 * h->n))) {
+        free_new();
+        return false;
+    }
+    const ColumnTiling from{h->TX, h->TY, h->nbx, h->nby}, to{tx, ty, c.nx / tx, c.ny / ty};
+    try {
+        for (int k = 0; k < 10; k++) {
+            void *dst = static_cast<char *>(base) + stride * k;
+            if (h->elem() == 4)
+                hipLaunchKernelGGL(retile_kernel<float>, dim3(4096), dim3(256), 0, h->stream,
+                                   (const float *)h->mom[k], (float *)dst, h->n, from, to);
+            else
+                hipLaunchKernelGGL(retile_kernel<_Float16>, dim3(4096), dim3(256), 0, h->stream,
+                                   (const _Float16 *)h->mom[k], (_Float16 *)dst, h->n, from, to);
+            HIPX(hipGetLastError());
+        }
+        for (int a = 0; uprev && a < 3; a++) {
+            hipLaunchKernelGGL(retile_kernel<float>, dim3(4096), dim3(256), 0, h->stream, h->uprev + a * h->n,
+                               uprev + a * h->n, h->n, from, to);
+            HIPX(hipGetLastError());
+        }
+        for (int s = 0; s < 2; s++) HIPX(hipMemsetAsync(faces[s], 0, sizeof(float) * fwords, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+    } catch (...) {
+        (void)hipStreamSynchronize(h->stream);
+        free_new();
+        throw;
+    }
+    (void)hipFree(h->mom_base);
+    for (auto &f : h->faces) (void)hipFree(f);
+    if (h->uprev) (void)hipFree(h->uprev);
+    h->mom_base = base;
+    h->mom_stride = stride;
+    for (int k = 0; k < 10; k++) h->mom[k] = static_cast<char *>(base) + stride * k;
+    h->faces[0] = faces[0];
+    h->faces[1] = faces[1];
+    if (uprev) h->uprev = uprev;
+    h->TX = tx;
+    h->TY = ty;
+    h->LZ = lz;
+    h->nbx = to.nbx;
+    h->nby = to.nby;
+    h->nc = c.nz / lz;
+    h->fwords = fwords;
+    h->yoff = yoff;
+    h->zoff = zoff;
+    h->perms.clear();
+    h->drop_graphs();
+    rebuild_faces(h);
+    HIPX(hipStreamSynchronize(h->stream));
+    return true;
+}
+
 int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     g_create_error.clear();
     if (!cfg || !out) {
@@ -567,13 +685,15 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     if (c.nx < 2 || c.ny < 2 || c.nz < 2) return bad(LBM_ERR_SIZE, "mesh dimensions must be >= 2");
     if (!(c.Re > 0) || !std::isfinite(c.u_inf)) return bad(LBM_ERR_ARG, "Re must be > 0 and u_inf finite");
     h->Q = c.velocity_set;
-    if (c.tile_x == 0 || c.tile_y == 0)
-        choose_tile(h->Q, c.nx, c.ny, c.storage == LBM_STORE_FP16, &h->TX, &h->TY);
-    else {
+    h->auto_tile = (c.tile_x == 0 || c.tile_y == 0) && c.tile_z <= 0;
+    if (c.tile_x == 0 || c.tile_y == 0) {
+        int lz;
+        auto_tiling(c, 0u, &h->TX, &h->TY, &lz);
+    } else {
         h->TX = c.tile_x;
         h->TY = c.tile_y;
     }
-    if (!tile_supported(h->TX, h->TY)) return bad(LBM_ERR_ARG, "unsupported tile footprint");
+    if (!tile_supported(h->Q, h->TX, h->TY)) return bad(LBM_ERR_ARG, "unsupported tile footprint");
     if (c.nx % h->TX || c.ny % h->TY)
         return bad(LBM_ERR_SIZE, "nx and ny must be multiples of the tile footprint " + std::to_string(h->TX) +
                                      "x" + std::to_string(h->TY));
@@ -583,7 +703,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     h->LZ = c.tile_z > 0 ? c.tile_z
                          : choose_lz(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY,
                                      resident_workgroups(h->Q, h->TX, h->TY, c.device),
-                                     c.storage == LBM_STORE_FP16);
+                                     c.storage == LBM_STORE_FP16, 0);
     if (c.nz % h->LZ) return bad(LBM_ERR_SIZE, "nz must be a multiple of tile_z");
     h->nbx = c.nx / h->TX;
     h->nby = c.ny / h->TY;
@@ -762,7 +882,13 @@ int lbm_fo_enable(lbm_handle *h, unsigned mask) {
         if (mask & ~31u) throw ArgError{LBM_ERR_ARG, "unknown function-object bits"};
         HIPX(hipStreamSynchronize(h->stream));
         h->drop_graphs();
-        layout_moments(h, mask, true);
+        bool retiled = false;
+        if (h->auto_tile) {
+            int tx, ty, lz;
+            auto_tiling(h->cfg, mask, &tx, &ty, &lz);
+            if (tx != h->TX || ty != h->TY || lz != h->LZ) retiled = retile(h, tx, ty, lz, mask);
+        }
+        if (!retiled) layout_moments(h, mask, true);
         const int64_t n = h->n;
         if ((mask & LBM_FO_K) && !h->k) dev_alloc((void **)&h->k, sizeof(float) * n, "k");
         if ((mask & LBM_FO_S) && !h->S) dev_alloc((void **)&h->S, sizeof(float) * 6 * n, "S");

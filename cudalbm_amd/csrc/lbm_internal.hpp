@@ -35,15 +35,16 @@ inline void relaxation(double Re, double u_inf, int nx, double *omega, double *t
     *omega = 1.0 / *tau;
 }
 
-bool tile_supported(int tx, int ty);
+// Footprints the single-case step is built for (D3Q19 also 32x32 and 64x16).
+bool tile_supported(int Q, int tx, int ty);
 void launch_step(int Q, int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool fo,
                  bool probe);
 void launch_fill(int Q, int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool probe);
 // Words of one face-buffer set and the offsets of its Y and Z blocks.
 int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_t *yoff, int64_t *zoff);
-int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool fp16);
+int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool fp16, int mean_bytes);
 int64_t resident_workgroups(int Q, int tx, int ty, int device);
-void choose_tile(int Q, int nx, int ny, bool fp16, int *tx, int *ty);
+void choose_tile(int Q, int nx, int ny, bool fp16, bool means, int *tx, int *ty);
 std::vector<int64_t> make_perm(int layout, int nx, int ny, int nz, int tx, int ty);
 
 }  // namespace lbm

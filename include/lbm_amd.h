@@ -91,7 +91,8 @@ const char *lbm_last_error(const lbm_handle *h);
 /* Message of the last failure of lbm_create (no handle exists then). */
 const char *lbm_create_error(void);
 
-/* Geometry actually chosen (after autotuning) and derived constants. */
+/* Geometry in use (chosen automatically unless set in lbm_config; with an
+ * automatic tiling lbm_fo_enable may change it) and derived constants. */
 int lbm_get_tiling(const lbm_handle *h, int *tile_x, int *tile_y, int *tile_z);
 int lbm_get_omega(const lbm_handle *h, double *omega, double *tau);
 
@@ -124,12 +125,15 @@ int lbm_set_time(lbm_handle *h, int64_t t);
  * (kineticEnergy.cuh:249).  The instantaneous fields (k, S) are functions of
  * the current moments alone, so they are formed from them when read, with the
  * same formulas (kineticEnergy.cuh:71-85, strainRateTensor.cuh:71-84).
- * Enabling resets the means to zero.  Synchronizes the handle's stream; turning
- * the means on or off may move the moment arrays inside device memory (a
- * layout measured faster for the step with the means), keeping their values:
- * the move briefly holds the moments twice (10 * n * element size more), and
- * when that memory is not available the arrays stay where they are.
- * LBM_ERR_NOMEM when an FO array cannot be allocated. */
+ * Enabling resets the means to zero.  Synchronizes the handle's stream.  Turning
+ * the means on or off may move the moment arrays inside device memory, keeping
+ * their values: with an automatic tiling (tile fields 0 at lbm_create) the case
+ * is re-tiled for the step with or without the means (D3Q19 with the means: a
+ * 32x32 footprint, 7% faster), otherwise only the arrays' spacing changes.  The
+ * move briefly holds the moments (and, re-tiling, the face buffers) twice; when
+ * that memory is not available the case stays as it is.  Results do not depend
+ * on the tiling (bit for bit).  LBM_ERR_NOMEM when an FO array cannot be
+ * allocated. */
 int lbm_fo_enable(lbm_handle *h, unsigned mask);
 /* Reads an FO field.  LBM_FIELD_K/KMEAN: n values; S/SMEAN: 6*n values in the
  * order xx,xy,xz,yy,yz,zz, component-major; MOMENTSMEAN: 10*n values, the means
@@ -150,7 +154,8 @@ int lbm_reduce(lbm_handle *h, double *kinetic_energy, double *residual);
  *   uprev        device buffer of uprev_count floats, owned by the caller,
  *                at least 3 * nodes: the previous velocity field in the
  *                library's internal order; read (unless `first`) and then
- *                overwritten with the current field.
+ *                overwritten with the current field.  lbm_fo_enable may change
+ *                that order: pass first = 1 on the next call after it.
  *   first        nonzero: no previous field yet; the residual is reported as 1
  *                (as on lbm_reduce's first call).
  *   out          device memory for 2 doubles: kinetic energy, residual.

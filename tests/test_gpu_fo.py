@@ -126,29 +126,45 @@ def test_layout_tiled8_round_trip(native_lib):
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 def test_fo_enable_keeps_the_moments(native_lib, storage):
-    """Turning the FP64 means on moves the D3Q19 moment arrays into the shifted
-    layout and turning them off moves them back; the moments survive both moves
-    bit for bit, and a run with both moves equals a run with none (the step is
-    deterministic and the layout changes no arithmetic)."""
+    """Turning the FP64 means on re-tiles an automatically tiled D3Q19 case
+    (32x16 -> 32x32 footprints, moments permuted on the device, face buffers
+    rebuilt) and turning them off moves it back; the moments survive both moves
+    bit for bit, a run with both moves equals a run with none (results do not
+    depend on the tiling), and the residual's stored previous field follows the
+    moves (FP64 sums over another node order: 1e-12 relative)."""
     from cudalbm_amd import Solver
     from cudalbm_amd import native as N
 
     n = 32
     with Solver(n, n, n, 400.0, 0.05, velocity_set=19, storage=storage) as s:
+        t0 = s.tiling
         s.step(30)
+        s.reduce()
         a = s.get_moments()
         s.fo_enable(N.LBM_FO_KMEAN | N.LBM_FO_SMEAN | N.LBM_FO_MOMENTSMEAN)
+        t1 = s.tiling
         b = s.get_moments()
         s.step(5)
+        ke1, r1 = s.reduce()
         c = s.get_moments()
         s.fo_enable(0)
+        t2 = s.tiling
         c2 = s.get_moments()
         s.step(5)
+        ke2, r2 = s.reduce()
         d = s.get_moments()
     with Solver(n, n, n, 400.0, 0.05, velocity_set=19, storage=storage) as s:
-        s.step(40)
+        s.step(30)
+        s.reduce()
+        s.step(5)
+        ke1r, r1r = s.reduce()
+        s.step(5)
+        ke2r, r2r = s.reduce()
         e = s.get_moments()
-    print(f"D3Q19 {storage}: moments kept across both layout changes", flush=True)
+    print(f"D3Q19 {storage}: tilings {t0} -> {t1} -> {t2}; moments kept across both moves", flush=True)
+    assert t1[:2] == (32, 32) and t0 == t2 and t0[:2] == (32, 16)
+    for x, y in ((ke1, ke1r), (r1, r1r), (ke2, ke2r), (r2, r2r)):
+        assert abs(x - y) <= 1e-12 * abs(y), (x, y)
     for k in range(10):
         assert np.array_equal(a[k], b[k]), k
         assert np.array_equal(c[k], c2[k]), k
