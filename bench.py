@@ -26,6 +26,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -50,28 +51,28 @@ def measured_traffic(key):
         return None
 
 
-def warm_vs_cold(n_draw=3, edge=128, max_steps=200000, re_steady=1500.0):
-    """Held-out cavities at edge^3 (the first n_draw of the evaluation draw, Re
-    log-uniform [400, 2500]) solved to residual 1e-6 from rest and from the
-    trained surrogate's prediction.  Only the cases with Re <= re_steady are
-    run: above it the cavity is not steady at this resolution (Re 1941 of the
-    draw keeps r ~ 1e-3 through 200k steps from either start)."""
-    import numpy as np
-
+def warm_vs_cold(n_cases=2, edge=128, max_steps=200000):
+    """The first n_cases edge^3 cavities of the held-out evaluation draw (64
+    cavities, a seed no training set uses, edges {128, 192}, Re log-uniform
+    [400, 2500]) solved to residual 1e-6 from rest and from the trained
+    surrogate's prediction.  Every case is reported; a case whose cold run does
+    not reach 1e-6 in max_steps (unsteady at this resolution) is flagged and
+    left out of the medians.  The whole 64-case evaluation is
+    `python -m cudalbm_amd.warmstart eval` (a run of many minutes)."""
     from cudalbm_amd import warmstart as W
 
     net, meta = W.load_model(MODEL_FILE)
     with net:
-        cases = [c for c in W.draw_cases(n_draw, W.EVAL_SEED, (edge,), re_range=(400.0, 2500.0)) if c[1] <= re_steady]
+        draw = W.draw_cases(64, W.HELDOUT_SEED, (128, 192), re_range=(400.0, 2500.0))
+        cases = [c for c in draw if c[0] == edge][:n_cases]
         res = W.evaluate(net, cases, max_steps=max_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
-    return {"cases": [{"N": r["N"], "Re": round(r["Re"], 1), "U": round(r["U"], 4),
+    return {"cases": [{"N": r["N"], "Re": round(r["Re"], 1), "U": round(r["U"], 4), "steady": r["steady"],
                        "cold_steps": r["cold"]["steps"], "cold_ms": round(r["cold"]["ms"], 1),
                        "cold_residual": r["cold"]["residual"], "warm_steps": r["warm"]["steps"],
                        "warm_ms": round(r["warm"]["ms"], 1), "warm_residual": r["warm"]["residual"],
                        "warm_kept": bool(r["warm"]["warm"]), "time_ratio": round(r["time_ratio"], 4),
                        "final_velocity_rel_l2": r["final_rel_l2"]} for r in res],
-            "median_time_ratio": float(np.median([r["time_ratio"] for r in res])),
-            "max_final_rel_l2": float(max(r["final_rel_l2"] for r in res)),
+            **W.summarise(res),
             "model": {"file": os.path.basename(MODEL_FILE),
                       **{k: meta[k] for k in ("S", "cases", "epochs", "final_loss") if k in meta}}}
 
@@ -131,39 +132,119 @@ def batch_sweep(steps, n_cases=64):
             "roofline_frac_batched": round(total * 80 / (ms_b * 1e-3 / steps) / 1e9 / PEAK_HBM_GBS, 4)}
 
 
-def cpu_baseline(target_s=25.0):
-    """FP64 oracle, D3Q19 96^3 cavity (Re 1000, lid 0.05) from rest, on this
-    host's cores; the step count is sized to about target_s seconds.  The
-    oracle step carries no function-object work, so the CPU figure is an upper
-    bound for the CPU on the headline workload."""
-    from oracle import oracle as O
-
-    n = 96
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    om = O.omega(0.05, 1000.0, n)
-    mom = O.rest_state(n, n, n)
-    O.run(19, n, n, n, om, 0.05, mom, 2, threads)  # first touch and thread start-up, untimed
-    t0 = time.perf_counter()
-    O.run(19, n, n, n, om, 0.05, mom, 10, threads)
-    per = (time.perf_counter() - t0) / 10
-    steps = max(2, int(target_s / max(per, 1e-6)))
-    t0 = time.perf_counter()
-    O.run(19, n, n, n, om, 0.05, mom, steps, threads)
-    dt = time.perf_counter() - t0
-    # one thread, a few steps, for the single-core figure
-    t1 = time.perf_counter()
-    O.run(19, n, n, n, om, 0.05, mom, 3, 1)
-    single = n ** 3 * 3 / (time.perf_counter() - t1) / 1e6
-    model = "unknown CPU"
+def _cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
-            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+            return next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
     except (OSError, StopIteration):
-        pass
-    return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, D3Q19 {n}^3 lid cavity Re 1000 u 0.05 from rest, {steps} steps, "
-                      f"{dt:.1f} s on {threads} threads of {os.cpu_count()} ({model}); "
-                      f"one thread: {single:.1f} MLUPS"}
+        return "unknown CPU"
+
+
+def _oracle_mlups(n, steps, threads, Re=1000.0, u=0.05):
+    """FP64 oracle on an n^3 D3Q19 cavity from rest: (MLUPS, seconds) of `steps`
+    steps after one untimed step (first touch, thread start-up)."""
+    from oracle import oracle as O
+
+    om = O.omega(u, Re, n)
+    mom = O.rest_state(n, n, n)
+    O.run(19, n, n, n, om, u, mom, 1, threads)
+    t0 = time.perf_counter()
+    O.run(19, n, n, n, om, u, mom, steps, threads)
+    dt = time.perf_counter() - t0
+    return n ** 3 * steps / dt / 1e6, dt
+
+
+def cpu_baseline(full_single=False):
+    """The FP64 oracle (the reference has no CPU path; this is its restatement)
+    on this host: D3Q19 cavities from rest at 64^3 x 1000 steps (Re 100, the
+    oracle's reference run) and at the benchmark size 256^3 x 20 steps (Re
+    1000), on all cores (up to 16 threads) and on one core (64^3 x 100 and
+    256^3 x 2 steps, or the full step counts with full_single).  The oracle
+    step carries no function-object work, so the figures bound the CPU from
+    above on the headline workload.  value = the all-core 256^3 MLUPS."""
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    m64, t64 = _oracle_mlups(64, 1000, threads, Re=100.0)
+    m256, t256 = _oracle_mlups(256, 20, threads)
+    s64, ts64 = _oracle_mlups(64, 1000 if full_single else 100, 1, Re=100.0)
+    s256, ts256 = _oracle_mlups(256, 20 if full_single else 2, 1)
+    ns = (1000, 20) if full_single else (100, 2)
+    return {"value": round(m256, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, D3Q19 lid cavity from rest: 256^3 x 20 steps (Re 1000) {m256:.1f} MLUPS "
+                      f"({t256:.1f} s) and 64^3 x 1000 steps (Re 100) {m64:.1f} MLUPS ({t64:.1f} s) on {threads} "
+                      f"threads; one thread: 256^3 x {ns[1]} {s256:.2f} MLUPS ({ts256:.1f} s), 64^3 x {ns[0]} "
+                      f"{s64:.2f} MLUPS ({ts64:.1f} s); host nproc {os.cpu_count()}, {_cpu_model()}",
+            "all_cores": {"d3q19_256_x20": round(m256, 2), "d3q19_64_x1000": round(m64, 2)},
+            "one_core": {f"d3q19_256_x{ns[1]}": round(s256, 3), f"d3q19_64_x{ns[0]}": round(s64, 3)}}
+
+
+def sweep_full(n_cases=64, tol=1e-6, check=100, max_steps=200000, segment=20000):
+    """Config 4 at its workload: the seeded 64-case sweep run by lbm_batch_run
+    until each case's FP64 residual is below tol or it took max_steps steps.
+    The run is split into segments of `segment` steps (cases stopped by a
+    segment's cap rejoin the next one; converged cases stay out), and prints
+    a progress line per segment.  Aggregate MLUPS = node updates of all cases /
+    wall time of all segments."""
+    import numpy as np
+
+    from cudalbm_amd import Batch
+    from cudalbm_amd.sweep import sweep_cases
+
+    cases = sweep_cases(n_cases)
+    nodes = np.array([c[0] * c[1] * c[2] for c in cases], dtype=np.int64)
+    res = np.full(n_cases, np.inf)
+    total_ms = 0.0
+    with Batch(cases) as b:
+        tiling = b.tiling
+        cap = 0
+        while cap < max_steps:
+            cap = min(cap + segment, max_steps)
+            steps, r, ms = b.run(tol, check, cap)
+            total_ms += ms
+            ran = np.isfinite(r)
+            res[ran] = r[ran]
+            again = (res >= tol) & (steps >= cap)
+            print(f"# sweep segment to {cap} steps: {ms / 1e3:.1f} s, {int((res < tol).sum())} converged, "
+                  f"{int(again.sum())} continue", file=sys.stderr, flush=True)
+            if not again.any():
+                break
+            b.set_active(again.astype(np.uint8))
+        steps = np.asarray(b.steps)
+    upd = int((steps * nodes).sum())
+    return {"cases": n_cases, "tol": tol, "check": check, "max_steps": max_steps, "tiling": list(tiling[:3]),
+            "persistent": bool(tiling[3]), "wall_s": round(total_ms / 1e3, 2), "node_updates": upd,
+            "mlups_batched": round(upd / (total_ms * 1e-3) / 1e6, 1),
+            "converged": int((res < tol).sum()), "steps": steps.tolist(),
+            "residual": [float(f"{v:.3e}") for v in res], "seed_cases": [list(c) for c in cases]}
+
+
+def sweep_sequential(steps_per_case, tol=1e-6, check=100):
+    """The same cases as one-after-another single-case solves to the same step
+    counts (the batched and single-case steps are bit-identical, so each case
+    stops at the same step): aggregate MLUPS over the whole sequence."""
+    from cudalbm_amd import Solver
+    from cudalbm_amd import native as NT
+    from cudalbm_amd.sweep import sweep_cases
+
+    cases = sweep_cases(len(steps_per_case))
+    total_ms, upd = 0.0, 0
+    for i, ((nx, ny, nz, Re, u), k) in enumerate(zip(cases, steps_per_case)):
+        with Solver(nx, ny, nz, Re, u) as s:
+            t0 = time.perf_counter()
+            info = NT.solve_to_residual(s, Re, u, tol, check, int(k))
+            dt = (time.perf_counter() - t0) * 1e3
+        total_ms += dt
+        upd += nx * ny * nz * int(info["steps"])
+        if i % 8 == 7:
+            print(f"# sequential: {i + 1} cases, {total_ms / 1e3:.1f} s", file=sys.stderr, flush=True)
+    return {"wall_s": round(total_ms / 1e3, 2), "node_updates": upd,
+            "mlups_sequential": round(upd / (total_ms * 1e-3) / 1e6, 1)}
 
 
 def main():
@@ -176,8 +257,11 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="only the headline D3Q19 FP32 step")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-vs-cold leg")
     ap.add_argument("--only", default="",
-                    help="profile one variant: q19fo|q19fp32|q19fp16|q19fp16fo|q27fp32|q27fp16|batch|warm")
+                    help="one leg: q19fo|q19fp32|q19fp16|q19fp16fo|q27fp32|q27fp16|batch|warm|sweep-full|sweep-seq")
     ap.add_argument("--tile", default="0,0,0")
+    ap.add_argument("--steps-json", default="", help="sweep-seq: JSON line of a sweep-full run (its per-case steps)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline only, with the one-core figures at the full step counts")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (x4), to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -198,7 +282,20 @@ def main():
     tile = tuple(int(v) for v in args.tile.split(","))
     FO_ALL = 15  # k | kMean | S | SMean
 
+    if args.cpu_full:
+        print(json.dumps({"cpu_baseline": cpu_baseline(full_single=True)}), flush=True)
+        return
     if args.only:
+        if args.only == "sweep-full":
+            r = sweep_full()
+            print(f"# batched: {r['mlups_batched']} MLUPS over {r['wall_s']} s", file=sys.stderr, flush=True)
+            print(json.dumps({"variant": "sweep-full", **r}), flush=True)
+            return
+        if args.only == "sweep-seq":
+            with open(args.steps_json) as f:
+                steps = json.load(f)["steps"]
+            print(json.dumps({"variant": "sweep-seq", **sweep_sequential(steps)}), flush=True)
+            return
         if args.only == "batch":
             print(json.dumps({"variant": "batch", **batch_sweep(args.steps)}), flush=True)
             return
@@ -254,7 +351,8 @@ def main():
                    "velocity_set": "D3Q19", "nodes": args.n ** 3, "tiling": head["tiling"]},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic(f"step_kernel<19,32,16,float,fo>@{args.n}^3"),
+                     "traffic": measured_traffic(f"step_kernel<19,{head['tiling'][0]},{head['tiling'][1]},float,fo>"
+                                                 f"@{args.n}^3"),
                      "bytes_per_node": head["bytes_per_node"]},
         "cpu_baseline": cpu,
         "variants": variants,

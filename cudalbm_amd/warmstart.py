@@ -34,6 +34,9 @@ from . import native as N
 
 TRAIN_SEED = 20261203
 EVAL_SEED = 20261204
+# held-out evaluation draws: a seed no training set uses (the training sets use
+# TRAIN_SEED and TRAIN_SEED + 1, which equals EVAL_SEED)
+HELDOUT_SEED = 20261207
 S_NET = 32
 # training edges: the sweep's range up to 128, weighted towards the cheap ones
 TRAIN_EDGES = (32, 48, 64, 96, 128)
@@ -159,22 +162,55 @@ def velocity_rel_l2(a, b):
     return float(np.linalg.norm(va - vb) / max(np.linalg.norm(vb), 1e-300))
 
 
-def evaluate(net, cases, tol=1e-6, check=100, max_steps=200000, log=print):
-    """Cold and warm solves of each (N, Re, U); returns a list of per-case dicts."""
+def evaluate(net, cases, tol=1e-6, check=100, max_steps=200000, log=print, converge_tol=None, converge_steps=0):
+    """Cold and warm solves of each (N, Re, U) to residual tol; returns a list of
+    per-case dicts.  With converge_tol, the cold run then continues to residual
+    converge_tol (at most converge_steps more steps) and both stopped fields are
+    also compared with that converged one.  A case whose cold run does not reach
+    tol within max_steps is marked unsteady."""
     out = []
     for n, Re, U in cases:
+        conv = None
         with N.Solver(n, n, n, Re, U) as s:
             cold = N.solve_to_residual(s, Re, U, tol, check, max_steps)
             mc = s.get_moments()
+            if converge_tol is not None and cold["residual"] < tol:
+                conv = N.solve_to_residual(s, Re, U, converge_tol, check, converge_steps)
+                mconv = s.get_moments()
         with N.Solver(n, n, n, Re, U) as s:
             warm = N.solve_to_residual(s, Re, U, tol, check, max_steps, surrogate=net)
             mw = s.get_moments()
         r = {"N": n, "Re": Re, "U": U, "cold": cold, "warm": warm, "final_rel_l2": velocity_rel_l2(mw, mc),
-             "time_ratio": warm["ms"] / cold["ms"], "step_ratio": warm["steps"] / max(cold["steps"], 1)}
+             "time_ratio": warm["ms"] / cold["ms"], "step_ratio": warm["steps"] / max(cold["steps"], 1),
+             "steady": bool(cold["residual"] < tol)}
+        if conv is not None:
+            r["converged"] = {"steps": int(cold["steps"] + conv["steps"]), "residual": conv["residual"]}
+            r["warm_vs_converged_rel_l2"] = velocity_rel_l2(mw, mconv)
+            r["cold_vs_converged_rel_l2"] = velocity_rel_l2(mc, mconv)
         log(f"# N {n} Re {Re:.0f} U {U:.3f}: cold {cold['steps']} steps {cold['ms']:.0f} ms (r {cold['residual']:.1e}), "
             f"warm {warm['steps']} steps {warm['ms']:.0f} ms (r {warm['residual']:.1e}, first {warm['first_residual']:.2e}, "
-            f"kept {warm['warm']}), time ratio {r['time_ratio']:.3f}, final rel-L2 {r['final_rel_l2']:.2e}")
+            f"kept {warm['warm']}), time ratio {r['time_ratio']:.3f}, final rel-L2 {r['final_rel_l2']:.2e}"
+            + (f", vs converged (r {conv['residual']:.1e}): warm {r['warm_vs_converged_rel_l2']:.2e} "
+               f"cold {r['cold_vs_converged_rel_l2']:.2e}" if conv is not None else "")
+            + ("" if r["steady"] else ", UNSTEADY (cold run did not reach the tolerance)"))
         out.append(r)
+    return out
+
+
+def summarise(res):
+    """Medians over the steady cases; unsteady ones counted separately."""
+    st = [r for r in res if r["steady"]]
+    out = {"cases": len(res), "steady": len(st), "unsteady": len(res) - len(st)}
+    if st:
+        out["median_time_ratio"] = float(np.median([r["time_ratio"] for r in st]))
+        out["max_time_ratio"] = float(max(r["time_ratio"] for r in st))
+        out["median_step_ratio"] = float(np.median([r["step_ratio"] for r in st]))
+        out["max_final_rel_l2"] = float(max(r["final_rel_l2"] for r in st))
+        conv = [r for r in st if "warm_vs_converged_rel_l2" in r]
+        if conv:
+            out["max_warm_vs_converged_rel_l2"] = float(max(r["warm_vs_converged_rel_l2"] for r in conv))
+            out["max_cold_vs_converged_rel_l2"] = float(max(r["cold_vs_converged_rel_l2"] for r in conv))
+    out["fell_back"] = int(sum(r["warm"]["fell_back"] for r in res))
     return out
 
 
@@ -228,7 +264,12 @@ def main(argv=None):
     e.add_argument("--edges", default="128")
     e.add_argument("--cases", type=int, default=4)
     e.add_argument("--max-steps", type=int, default=200000)
-    e.add_argument("--seed", type=int, default=EVAL_SEED)
+    e.add_argument("--seed", type=int, default=HELDOUT_SEED)
+    e.add_argument("--first", type=int, default=0, help="evaluate cases first .. first+count-1 of the draw")
+    e.add_argument("--count", type=int, default=0, help="number of cases of the slice (0: all)")
+    e.add_argument("--converge-tol", type=float, default=0.0,
+                   help="continue each cold run to this residual and compare both fields with it (0: off)")
+    e.add_argument("--converge-steps", type=int, default=200000)
     e.add_argument("--re-range", default="400,2500", help="Re interval of the log-uniform draw")
     e.add_argument("--json", default="")
     a = ap.parse_args(argv)
@@ -276,12 +317,11 @@ def main(argv=None):
         edges = tuple(int(v) for v in a.edges.split(","))
         lo, hi = (float(v) for v in a.re_range.split(","))
         cases = draw_cases(a.cases, a.seed, edges, re_range=(lo, hi))
-        res = evaluate(net, cases, max_steps=a.max_steps, log=log)
-        summary = {"cases": len(res),
-                   "median_time_ratio": float(np.median([r["time_ratio"] for r in res])),
-                   "median_step_ratio": float(np.median([r["step_ratio"] for r in res])),
-                   "max_final_rel_l2": float(max(r["final_rel_l2"] for r in res)),
-                   "fell_back": int(sum(r["warm"]["fell_back"] for r in res)), "per_case": res}
+        cases = cases[a.first:a.first + a.count] if a.count else cases[a.first:]
+        res = evaluate(net, cases, max_steps=a.max_steps, log=log,
+                       converge_tol=a.converge_tol or None, converge_steps=a.converge_steps)
+        summary = {**summarise(res), "seed": a.seed, "first": a.first, "edges": list(edges),
+                   "re_range": [lo, hi], "model": os.path.basename(a.model), "per_case": res}
         if a.json:
             with open(a.json, "w") as f:
                 json.dump(summary, f)
