@@ -6,16 +6,14 @@
 
 namespace lbm {
 
-// FP32 staging <-> FP16 encoded moments.
-static __global__ void encode_f16_kernel(const float *__restrict__ src, _Float16 *__restrict__ dst, int64_t n,
-                                  float enc) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = _Float16(src[i] * enc);
+// FP32 staging (internal node order) <-> FP16 encoded moment k (p's layout).
+static __global__ void encode_f16_kernel(const StepParams p, int k, const float *__restrict__ src, float enc) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < p.n; i += (int64_t)gridDim.x * blockDim.x)
+        Store<_Float16>::store(p.mom[k], mom_off<_Float16>(p, i), _Float16(src[i] * enc));
 }
-static __global__ void decode_f16_kernel(const _Float16 *__restrict__ src, float *__restrict__ dst, int64_t n,
-                                  float dec) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = float(src[i]) * dec;
+static __global__ void decode_f16_kernel(const StepParams p, int k, float *__restrict__ dst, float dec) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < p.n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = Store<_Float16>::load(p.mom[k], mom_off<_Float16>(p, i), dec);
 }
 
 static __global__ void fill_kernel_f64(double *p, int64_t n, double v) {
@@ -32,8 +30,9 @@ template <typename ST>
 __global__ void fo_instant_kernel(const StepParams p, float *__restrict__ k, float *__restrict__ S) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < p.n; i += (int64_t)gridDim.x * blockDim.x) {
         float m[10];
+        const unsigned off = mom_off<ST>(p, i);
 #pragma unroll
-        for (int c = 0; c < 10; c++) m[c] = Store<ST>::load(p.mom[c], (unsigned)i * Store<ST>::kBytes, p.dec[c]);
+        for (int c = 0; c < 10; c++) m[c] = Store<ST>::load(p.mom[c], off, p.dec[c]);
         float kk, s[6];
         fo_values(m, p.s_diag, p.s_off, kk, s);
         if (k) k[i] = kk;
@@ -69,10 +68,11 @@ __device__ __forceinline__ void residual_partials(const StepParams &p, float *__
     double ke = 0.0, du = 0.0, uu = 0.0;
     for (int64_t i = j * (int64_t)NT + threadIdx.x; i < p.n; i += (int64_t)nb * NT) {
         // physical rho and u formed in FP64 from the stored values
-        const double rho = 1.0 + (double)Store<ST>::load(p.mom[0], (unsigned)i * Store<ST>::kBytes, p.dec[0]);
+        const unsigned off = mom_off<ST>(p, i);
+        const double rho = 1.0 + (double)Store<ST>::load(p.mom[0], off, p.dec[0]);
         double u[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) u[a] = (double)Store<ST>::load(p.mom[1 + a], (unsigned)i * Store<ST>::kBytes, p.dec[1 + a]) / 3.0;
+        for (int a = 0; a < 3; a++) u[a] = (double)Store<ST>::load(p.mom[1 + a], off, p.dec[1 + a]) / 3.0;
         double uu_i = 0.0, du_i = 0.0;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -121,16 +121,18 @@ __global__ __launch_bounds__(NT) void residual_final_kernel(const double *__rest
 
 // Column-tiled node order of one footprint: node (x, y, z) at
 // ((z nby + by) nbx + bx) tx ty + (y mod ty) tx + (x mod tx).
+// mplane: elements from one plane block of an array to its next (tx ty for a
+// plain array, 10 tx ty for the interleaved moment arrays).
 struct ColumnTiling {
-    int tx, ty, nbx, nby;
-    __device__ __forceinline__ int64_t index(int x, int y, int z) const {
+    int tx, ty, nbx, nby, mplane;
+    __device__ __forceinline__ int64_t element(int x, int y, int z) const {
         const int bx = x / tx, by = y / ty;
-        return (((int64_t)z * nby + by) * nbx + bx) * (int64_t)(tx * ty) + (int64_t)(y % ty) * tx + (x % tx);
+        return (((int64_t)z * nby + by) * nbx + bx) * (int64_t)mplane + (int64_t)(y % ty) * tx + (x % tx);
     }
 };
 
-// One array of n elements from the node order `from` into the order `to`
-// (a permutation: dst[to(x)] = src[from(x)]); grid-stride over the destination.
+// One array of n nodes from the node order `from` into the order `to`
+// (a permutation: dst[to(x)] = src[from(x)]); grid-stride over the destination nodes.
 template <typename T>
 __global__ void retile_kernel(const T *__restrict__ src, T *__restrict__ dst, int64_t n, ColumnTiling from,
                               ColumnTiling to) {
@@ -140,7 +142,7 @@ __global__ void retile_kernel(const T *__restrict__ src, T *__restrict__ dst, in
         const int z = (int)(tile / cols), c = (int)(tile % cols);
         const int by = c / to.nbx, bx = c % to.nbx;
         const int y = by * to.ty + (int)(r / to.tx), x = bx * to.tx + (int)(r % to.tx);
-        dst[i] = src[from.index(x, y, z)];
+        dst[to.element(x, y, z)] = src[from.element(x, y, z)];
     }
 }
 

@@ -136,6 +136,9 @@ struct Case {
         p.u_lid = (float)c.u_inf;
         p.tau = (float)tau;
         for (int k = 0; k < 10; k++) p.mom[k] = mom + (int64_t)k * n;
+        // ten separate arrays: plane blocks of tx ty nodes, contiguous
+        p.mplane = (c.nx / nbx) * (c.ny / nby);
+        p.mshift = __builtin_ctz((unsigned)p.mplane);
         p.fin = faces + (int64_t)set * fwords;
         p.fout = faces + (int64_t)(set ^ 1) * fwords;
         p.yoff = (int)yoff;

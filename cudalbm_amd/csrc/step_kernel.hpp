@@ -56,7 +56,14 @@ struct StepClock {
 struct StepParams {
     int nx, ny, nz, lz, nbx, nby, nc;
     float omega, t_omega, omega_h, u_lid, tau;
-    void *mom[10];         // stored moments, column-tiled internal layout, in place
+    // Stored moments, column-tiled internal layout, in place.  Node i of the
+    // internal order (plane block b = i / (tx ty), r = i mod (tx ty)) of moment k
+    // is element b * mplane + r of mom[k]: mplane = tx ty for ten separate
+    // arrays, 10 tx ty when the ten arrays are interleaved per plane block
+    // (mom[k] = base + k tx ty elements; every plane block of a workgroup then
+    // is one contiguous run of all ten moments).  mshift = log2(tx ty).
+    void *mom[10];
+    int mplane, mshift;
     const float *fin;      // face buffers read (previous step): X | Y | Z
     float *fout;           // face buffers written (this step)
     int yoff, zoff;        // offsets of the Y and Z face blocks inside one face set
@@ -415,10 +422,26 @@ __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__res
         write_x_faces<Q, TX, TY, PROBE>(p, fout, r, tx, x, y, z, bx);
 }
 
-// First node of this workgroup's tile in plane z (uniform); add the thread index.
+// Plane block of this workgroup's tile in plane z (uniform).
+__device__ __forceinline__ unsigned plane_block(const StepParams &p, int bx, int by, int z) {
+    return (unsigned)((z * p.nby + by) * p.nbx + bx);
+}
+// First node of the block in the internal node order (the FP64 means, k, S
+// and the residual fields are indexed by it); add the thread index.
 template <int TX, int TY>
 __device__ __forceinline__ unsigned plane_base(const StepParams &p, int bx, int by, int z) {
-    return (unsigned)(((z * p.nby + by) * p.nbx + bx) * (TX * TY));
+    return plane_block(p, bx, by, z) * (unsigned)(TX * TY);
+}
+// Byte offset of the block's first node in every moment array.
+template <typename ST>
+__device__ __forceinline__ unsigned mom_plane_off(const StepParams &p, int bx, int by, int z) {
+    return plane_block(p, bx, by, z) * (unsigned)p.mplane * Store<ST>::kBytes;
+}
+// Byte offset of internal node i in every moment array.
+template <typename ST>
+__device__ __forceinline__ unsigned mom_off(const StepParams &p, int64_t i) {
+    const unsigned u = (unsigned)i;
+    return ((u >> p.mshift) * (unsigned)p.mplane + (u & ((1u << p.mshift) - 1u))) * Store<ST>::kBytes;
 }
 
 // The same plane as stored (FP16: not yet scaled by the codec's decode
@@ -426,7 +449,7 @@ __device__ __forceinline__ unsigned plane_base(const StepParams &p, int bx, int 
 template <typename ST, int TX, int TY>
 __device__ __forceinline__ void load_plane_raw(const StepParams &p, int bx, int by, int z, int tid,
                                                float (&m)[10]) {
-    const unsigned off = (plane_base<TX, TY>(p, bx, by, z) + (unsigned)tid) * Store<ST>::kBytes;
+    const unsigned off = mom_plane_off<ST>(p, bx, by, z) + (unsigned)tid * Store<ST>::kBytes;
 #pragma unroll
     for (int k = 0; k < 10; k++) m[k] = Store<ST>::load_raw(p.mom[k], off);
 }
