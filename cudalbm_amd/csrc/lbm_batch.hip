@@ -113,7 +113,7 @@ struct Case {
     int lz = 1, nbx = 0, nby = 0, nc = 0;
     int64_t n = 0, fwords = 0, yoff = 0, zoff = 0;
     double omega = 0, tau = 0;
-    float *mom = nullptr;    // 10 arrays of n floats
+    float *mom = nullptr;    // the 10 moments of n nodes: interleaved per plane block (mplane())
     float *faces = nullptr;  // 2 face sets of fwords floats
     float *uprev = nullptr;  // 3 arrays of n floats (residual reference field)
     int64_t t = 0;
@@ -135,10 +135,11 @@ struct Case {
         p.omega_h = (float)(omega * 0.5);
         p.u_lid = (float)c.u_inf;
         p.tau = (float)tau;
-        for (int k = 0; k < 10; k++) p.mom[k] = mom + (int64_t)k * n;
-        // ten separate arrays: plane blocks of tx ty nodes, contiguous
-        p.mplane = (c.nx / nbx) * (c.ny / nby);
-        p.mshift = __builtin_ctz((unsigned)p.mplane);
+        // interleaved per plane block as a single case's moments (StepParams::mplane),
+        // or ten separate arrays where 32-bit byte offsets would not reach
+        for (int k = 0; k < 10; k++) p.mom[k] = mom + (int64_t)k * kstride();
+        p.mplane = mplane();
+        p.mshift = __builtin_ctz((unsigned)nt());
         p.fin = faces + (int64_t)set * fwords;
         p.fout = faces + (int64_t)(set ^ 1) * fwords;
         p.yoff = (int)yoff;
@@ -148,6 +149,21 @@ struct Case {
         return p;
     }
     int grid() const { return nbx * nby * nc; }
+    int nt() const { return (c.nx / nbx) * (c.ny / nby); }
+    bool interleaved() const { return sizeof(float) * 10 * (size_t)n <= ((size_t)1 << 32); }
+    int mplane() const { return interleaved() ? 10 * nt() : nt(); }
+    int64_t kstride() const { return interleaved() ? nt() : n; }  // elements from mom[k] to mom[k+1]
+    // one moment between a host array in the internal node order and the device
+    void h2d(int k, const float *src, hipStream_t s) const {
+        const size_t w = sizeof(float) * nt();
+        HIPX(hipMemcpy2DAsync(mom + (int64_t)k * kstride(), sizeof(float) * mplane(), src, w, w, (size_t)n / nt(),
+                              hipMemcpyHostToDevice, s));
+    }
+    void d2h(int k, float *dst, hipStream_t s) const {
+        const size_t w = sizeof(float) * nt();
+        HIPX(hipMemcpy2DAsync(dst, w, mom + (int64_t)k * kstride(), sizeof(float) * mplane(), w, (size_t)n / nt(),
+                              hipMemcpyDeviceToHost, s));
+    }
 };
 
 }  // namespace
@@ -688,8 +704,7 @@ int lbm_batch_set_moments(lbm_batch *b, int cs, const float *const m[10], int la
         for (int k = 0; k < 10; k++) {
             if (!m[k]) throw ArgError{LBM_ERR_ARG, "null moment array"};
             for (int64_t u = 0; u < c.n; u++) tmp[(size_t)perm[(size_t)u]] = m[k][u];
-            HIPX(hipMemcpyAsync(c.mom + (int64_t)k * c.n, tmp.data(), sizeof(float) * c.n, hipMemcpyHostToDevice,
-                                b->stream));
+            c.h2d(k, tmp.data(), b->stream);
             HIPX(hipStreamSynchronize(b->stream));
         }
         fill_faces(b, cs);
@@ -708,8 +723,7 @@ int lbm_batch_get_moments(lbm_batch *b, int cs, float *m[10], int layout) {
         std::vector<float> tmp((size_t)c.n);
         for (int k = 0; k < 10; k++) {
             if (!m[k]) throw ArgError{LBM_ERR_ARG, "null moment array"};
-            HIPX(hipMemcpyAsync(tmp.data(), c.mom + (int64_t)k * c.n, sizeof(float) * c.n, hipMemcpyDeviceToHost,
-                                b->stream));
+            c.d2h(k, tmp.data(), b->stream);
             HIPX(hipStreamSynchronize(b->stream));
             for (int64_t u = 0; u < c.n; u++) m[k][u] = tmp[(size_t)perm[(size_t)u]];
         }

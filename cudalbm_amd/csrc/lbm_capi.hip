@@ -288,7 +288,8 @@ struct lbm_handle {
     int parity = 0;
     unsigned fo_mask = 0;
     float *k = nullptr, *S = nullptr;
-    double *kmean = nullptr, *smean = nullptr, *mmean = nullptr;
+    double *mean_buf = nullptr;  // the enabled FP64 means, interleaved per plane block (StepParams)
+    int mean_comp = 0;        // components in `means`
     StepClock *clock = nullptr;
     int64_t t = 0;
     float *uprev = nullptr;
@@ -330,9 +331,14 @@ struct lbm_handle {
         p.zoff = zoff;
         codec(p.enc, p.dec);
         p.fo_mask = probe ? 0u : fo_mask;
-        p.kmean = kmean;
-        p.smean = smean;
-        p.mmean = mmean;
+        const int nt = TX * TY;
+        int c = 0;
+        p.kmean = (fo_mask & LBM_FO_KMEAN) ? mean_buf + (c++) * (size_t)nt : nullptr;
+        p.smean = (fo_mask & LBM_FO_SMEAN) ? mean_buf + c * (size_t)nt : nullptr;
+        c += (fo_mask & LBM_FO_SMEAN) ? 6 : 0;
+        p.mmean = (fo_mask & LBM_FO_MOMENTSMEAN) ? mean_buf + c * (size_t)nt : nullptr;
+        p.fcomp = nt;
+        p.fplane = mean_comp * nt;
         p.clock = clock + parity;
         p.clock_next = clock + (parity ^ 1);
         p.s_diag = (float)(1.0 / (3.0 * tau));
@@ -366,11 +372,12 @@ struct lbm_handle {
             if (f) (void)hipFree(f);
             f = nullptr;
         }
-        for (void *p : {(void *)k, (void *)S, (void *)kmean, (void *)smean, (void *)mmean, (void *)clock,
-                        (void *)uprev, (void *)part, (void *)red_out})
+        for (void *p : {(void *)k, (void *)S, (void *)mean_buf, (void *)clock, (void *)uprev, (void *)part,
+                        (void *)red_out})
             if (p) (void)hipFree(p);
         k = S = nullptr;
-        kmean = smean = mmean = nullptr;
+        mean_buf = nullptr;
+        mean_comp = 0;
         clock = nullptr;
         uprev = nullptr;
         part = red_out = nullptr;
@@ -904,18 +911,20 @@ int lbm_fo_enable(lbm_handle *h, unsigned mask) {
         const int64_t n = h->n;
         if ((mask & LBM_FO_K) && !h->k) dev_alloc((void **)&h->k, sizeof(float) * n, "k");
         if ((mask & LBM_FO_S) && !h->S) dev_alloc((void **)&h->S, sizeof(float) * 6 * n, "S");
-        if ((mask & LBM_FO_KMEAN) && !h->kmean) dev_alloc((void **)&h->kmean, sizeof(double) * n, "kMean");
-        if ((mask & LBM_FO_SMEAN) && !h->smean) dev_alloc((void **)&h->smean, sizeof(double) * 6 * n, "SMean");
-        if ((mask & LBM_FO_MOMENTSMEAN) && !h->mmean)
-            dev_alloc((void **)&h->mmean, sizeof(double) * 10 * n, "momentsMean");
-        const std::pair<unsigned, std::pair<double *, int64_t>> zero[3] = {
-            {LBM_FO_KMEAN, {h->kmean, n}}, {LBM_FO_SMEAN, {h->smean, 6 * n}}, {LBM_FO_MOMENTSMEAN, {h->mmean, 10 * n}}};
-        for (const auto &z : zero)
-            if (mask & z.first) {
-                hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, z.second.first,
-                                   z.second.second, 0.0);
-                HIPX(hipGetLastError());
-            }
+        // the means' components, interleaved per plane block; reset to zero
+        const int comp = ((mask & LBM_FO_KMEAN) ? 1 : 0) + ((mask & LBM_FO_SMEAN) ? 6 : 0) +
+                         ((mask & LBM_FO_MOMENTSMEAN) ? 10 : 0);
+        if (comp != h->mean_comp) {
+            if (h->mean_buf) (void)hipFree(h->mean_buf);
+            h->mean_buf = nullptr;
+            h->mean_comp = 0;
+            if (comp) dev_alloc((void **)&h->mean_buf, sizeof(double) * comp * n, "the FP64 means");
+            h->mean_comp = comp;
+        }
+        if (comp) {
+            hipLaunchKernelGGL(fill_kernel_f64, dim3(2048), dim3(256), 0, h->stream, h->mean_buf, (int64_t)comp * n, 0.0);
+            HIPX(hipGetLastError());
+        }
         set_clock(h);
         HIPX(hipStreamSynchronize(h->stream));
         h->fo_mask = mask;
@@ -954,8 +963,15 @@ int lbm_fo_get(lbm_handle *h, int field, double *out, int layout) {
                            hipMemcpyDeviceToHost));
             for (size_t i = 0; i < f.size(); i++) tmp[i] = f[i];
         } else {
-            const double *src = field == LBM_FIELD_KMEAN ? h->kmean : field == LBM_FIELD_SMEAN ? h->smean : h->mmean;
-            HIPX(hipMemcpy(tmp.data(), src, sizeof(double) * n * ncomp, hipMemcpyDeviceToHost));
+            // de-interleave the field's components from the plane blocks
+            std::vector<double> all((size_t)n * h->mean_comp);
+            HIPX(hipMemcpy(all.data(), h->mean_buf, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+            const StepParams p = h->params();
+            const double *first = field == LBM_FIELD_KMEAN ? p.kmean : field == LBM_FIELD_SMEAN ? p.smean : p.mmean;
+            const int64_t c0 = (first - h->mean_buf) / p.fcomp, nt = p.fcomp;
+            for (int c = 0; c < ncomp; c++)
+                for (int64_t i = 0; i < n; i++)
+                    tmp[(size_t)(c * n + i)] = all[(size_t)((i / nt) * p.fplane + (c0 + c) * nt + i % nt)];
         }
         for (int c = 0; c < ncomp; c++)
             for (int64_t u = 0; u < n; u++) out[c * n + u] = tmp[(size_t)(c * n + perm[(size_t)u])];

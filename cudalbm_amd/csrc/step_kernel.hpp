@@ -71,9 +71,15 @@ struct StepParams {
     // fused FP64 time means of the function objects (fo_mask bits 2: kMean,
     // 8: SMean, 16: momentsMean)
     unsigned fo_mask;
+    // The enabled means' components (kMean; SMean xx..zz; momentsMean of the ten
+    // stored moments) are interleaved per plane block in one allocation:
+    // component c of node r of plane block b at b * fplane + c * fcomp + r, with
+    // fcomp = tx ty and fplane = (number of enabled components) tx ty.  kmean,
+    // smean, mmean point at the first component of each mean (null when off).
     double *kmean;
-    double *smean;         // 6 component arrays, each of n doubles
-    double *mmean;         // 10 component arrays (means of the stored moments), each of n doubles
+    double *smean;
+    double *mmean;
+    int fplane, fcomp;
     const StepClock *clock; // absolute step index of this step and 1/(t+1) (device)
     StepClock *clock_next;  // the next step's clock, or null when nothing reads it
     float s_diag, s_off;    // strain-rate coefficients 1/(3 tau), 1/(6 tau)
@@ -114,6 +120,7 @@ struct Store<float> {
         return __builtin_nontemporal_load(static_cast<const float *>(byte_at(p, off)));
     }
     __device__ static __forceinline__ float load_raw(const void *p, unsigned off) { return load(p, off, 1.f); }
+    __device__ static __forceinline__ float raw_word(float v) { return v; }
     __device__ static __forceinline__ void store(void *p, unsigned off, float v) {
         __builtin_nontemporal_store(v, static_cast<float *>(byte_at(p, off)));
     }
@@ -124,9 +131,17 @@ struct Store<_Float16> {
     __device__ static __forceinline__ float load(const void *p, unsigned off, float dec) {
         return float(__builtin_nontemporal_load(static_cast<const _Float16 *>(byte_at(p, off)))) * dec;
     }
-    // the stored value only: the step folds the decode scales into rec_setup_st
+    // The stored code as a raw word (the 16-bit code in the low half of a
+    // 32-bit word, as the load leaves it): rec_setup_st decodes it with
+    // mixed-precision FMAs that take the FP16 operand directly.
     __device__ static __forceinline__ float load_raw(const void *p, unsigned off) {
-        return float(__builtin_nontemporal_load(static_cast<const _Float16 *>(byte_at(p, off))));
+        return raw_word(__builtin_nontemporal_load(static_cast<const _Float16 *>(byte_at(p, off))));
+    }
+    __device__ static __forceinline__ float raw_word(_Float16 v) {
+        return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, v));
+    }
+    __device__ static __forceinline__ float from_raw(float w) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)__float_as_uint(w));
     }
     __device__ static __forceinline__ void store(void *p, unsigned off, _Float16 v) {
         __builtin_nontemporal_store(v, static_cast<_Float16 *>(byte_at(p, off)));
@@ -367,31 +382,36 @@ struct Edge {
     }
 };
 
-// Population-rebuild terms from moments as stored (load_plane_raw, the edge
-// codes of flush_xy_faces): FP32 as
-// rec_setup; FP16 with the decode factors folded into the products (they are
-// equal within the groups {1,2,3}, {4,7,9}, {5,6,8}: the codec's u, Pi_aa and
-// Pi_ab scales).
+// Population-rebuild terms from moments as stored: the raw words of
+// load_plane_raw / Store::raw_word.  FP32: rec_setup.  FP16: decoded by
+// mixed-precision FMAs, FP16 operand taken directly (no separate conversion);
+// the decode factors are equal within the groups {1,2,3}, {4,7,9}, {5,6,8} (the
+// codec's u, Pi_aa and Pi_ab scales) and the density code is 3 (rho - 1) itself.
+// Every population the step streams or a face buffer holds is rebuilt by this
+// one function, so all readers of a stored state see the same bits.
 template <typename ST>
 __device__ __forceinline__ Rec rec_setup_st(const float (&h)[10]) {
     if constexpr (sizeof(ST) == 4) {
         return rec_setup(h);
     } else {
+        auto hv = [&](int k) { return Store<ST>::from_raw(h[k]); };
         Rec r;
-        const float m0 = h[0] * codec_dec(0);
-        const float rho = 1.0f + m0;
-        r.a = fmaf(rho, -(h[4] + h[7] + h[9]) * codec_dec(4), 3.0f * m0);
+        const float rho = fmaf(hv(0), codec_dec(0), 1.0f);
+        float sd = fmaf(hv(9), codec_dec(4), 0.0f);
+        sd = fmaf(hv(7), codec_dec(4), sd);
+        sd = fmaf(hv(4), codec_dec(4), sd);
+        r.a = fmaf(rho, -sd, hv(0));  // 3 rho' - rho (mxx + myy + mzz)
         const float t = 3.0f * rho;
         const float tu = t * codec_dec(1), td = t * codec_dec(4), to = t * codec_dec(5);
-        r.M[1] = tu * h[1];
-        r.M[2] = tu * h[2];
-        r.M[3] = tu * h[3];
-        r.M[4] = td * h[4];
-        r.M[7] = td * h[7];
-        r.M[9] = td * h[9];
-        r.M[5] = to * h[5];
-        r.M[6] = to * h[6];
-        r.M[8] = to * h[8];
+        r.M[1] = fmaf(hv(1), tu, 0.0f);
+        r.M[2] = fmaf(hv(2), tu, 0.0f);
+        r.M[3] = fmaf(hv(3), tu, 0.0f);
+        r.M[4] = fmaf(hv(4), td, 0.0f);
+        r.M[7] = fmaf(hv(7), td, 0.0f);
+        r.M[9] = fmaf(hv(9), td, 0.0f);
+        r.M[5] = fmaf(hv(5), to, 0.0f);
+        r.M[6] = fmaf(hv(6), to, 0.0f);
+        r.M[8] = fmaf(hv(8), to, 0.0f);
         r.M[0] = 0.f;
         return r;
     }
@@ -413,7 +433,7 @@ __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__res
     E::coords(tid, tx, ty);
     float m[10];
 #pragma unroll
-    for (int k = 0; k < 10; k++) m[k] = float(edge_m[k * E::NE + tid]);
+    for (int k = 0; k < 10; k++) m[k] = Store<ST>::raw_word(edge_m[k * E::NE + tid]);
     const Rec r = rec_setup_st<ST>(m);
     const int x = bx * TX + tx, y = by * TY + ty;
     if (tid < 2 * TX)
