@@ -146,6 +146,64 @@ __global__ void retile_kernel(const T *__restrict__ src, T *__restrict__ dst, in
     }
 }
 
+// Warm start: a coarse S^3 field (channels u/U, v/U, w/U, p'/U^2 with
+// p' = (rho - 1)/3, channel c of point q at f[q * cstride + c]) interpolated
+// trilinearly onto the N^3 nodes (network node i at solver coordinate
+// i (N-1)/(S-1)) and written as the equilibrium moments in the solver's
+// layout: rho - 1 = 3 U^2 p', 3u, 4.5 u^2, 9 uv, ...; walls at rest, the lid
+// row moving at U along +x.  One thread per node of the internal order.
+template <typename ST>
+__global__ void warm_moments_kernel(const StepParams p, ColumnTiling t, const float *__restrict__ f, int S,
+                                    int cstride, double U) {
+    const int N = p.nx;
+    const int64_t per = (int64_t)t.tx * t.ty, cols = (int64_t)t.nbx * t.nby;
+    auto axis = [&](int j, int &k, float &fr) {
+        const double sc = N > 1 ? (double)j * (S - 1) / (N - 1) : 0.0;
+        k = min(S - 2, (int)floor(sc));
+        fr = (float)(sc - k);
+    };
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < p.n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t tile = i / per, r = i % per;
+        const int z = (int)(tile / cols), c = (int)(tile % cols);
+        const int y = (c / t.nbx) * t.ty + (int)(r / t.tx), x = (c % t.nbx) * t.tx + (int)(r % t.tx);
+        int kx, ky, kz;
+        float fx, fy, fz;
+        axis(x, kx, fx);
+        axis(y, ky, fy);
+        axis(z, kz, fz);
+        float val[4];
+        for (int ch = 0; ch < 4; ch++) {
+            float acc = 0.f;
+            for (int d = 0; d < 8; d++) {
+                const int ix = kx + (d & 1), iy = ky + ((d >> 1) & 1), iz = kz + (d >> 2);
+                const float wx = (d & 1) ? fx : 1.f - fx;
+                const float wy = ((d >> 1) & 1) ? fy : 1.f - fy;
+                const float wz = (d >> 2) ? fz : 1.f - fz;
+                acc += wx * wy * wz * f[(((int64_t)iz * S + iy) * S + ix) * cstride + ch];
+            }
+            val[ch] = acc;
+        }
+        double u = U * val[0], v = U * val[1], w = U * val[2];
+        const double rm1 = 3.0 * U * U * val[3];
+        if (y == N - 1) {
+            u = U;
+            v = w = 0.0;
+        } else if (x == 0 || x == N - 1 || y == 0 || z == 0 || z == N - 1) {
+            u = v = w = 0.0;
+        }
+        const float m[10] = {(float)rm1, (float)(3.0 * u), (float)(3.0 * v), (float)(3.0 * w), (float)(4.5 * u * u),
+                             (float)(9.0 * u * v), (float)(9.0 * u * w), (float)(4.5 * v * v), (float)(9.0 * v * w),
+                             (float)(4.5 * w * w)};
+        const unsigned off = mom_off<ST>(p, i);
+        for (int k = 0; k < 10; k++) {
+            if constexpr (sizeof(ST) == 4)
+                Store<ST>::store(p.mom[k], off, m[k]);
+            else
+                Store<ST>::store(p.mom[k], off, _Float16(m[k] * p.enc[k]));
+        }
+    }
+}
+
 // Kinetic energy and residual r = sqrt(sum |u - u_prev|^2 / sum |u|^2) from the
 // three sums of residual_final_kernel, on the device (one thread).
 static __global__ void residual_result_kernel(const double *__restrict__ sums, double *__restrict__ out) {

@@ -552,6 +552,21 @@ void do_steps(lbm_handle *h, int n) {
 
 }  // namespace
 
+void lbm::warm_moments(lbm_handle *h, const float *field, int S, int cstride, double U) {
+    HIPX(hipSetDevice(h->cfg.device));
+    HIPX(hipStreamSynchronize(h->stream));
+    const StepParams p = h->params();
+    const ColumnTiling t{h->TX, h->TY, h->nbx, h->nby, h->TX * h->TY};
+    if (h->cfg.storage == LBM_STORE_FP32)
+        hipLaunchKernelGGL(warm_moments_kernel<float>, dim3(4096), dim3(256), 0, h->stream, p, t, field, S, cstride, U);
+    else
+        hipLaunchKernelGGL(warm_moments_kernel<_Float16>, dim3(4096), dim3(256), 0, h->stream, p, t, field, S, cstride,
+                           U);
+    HIPX(hipGetLastError());
+    rebuild_faces(h);
+    HIPX(hipStreamSynchronize(h->stream));
+}
+
 extern "C" {
 
 const char *lbm_create_error(void) { return g_create_error.c_str(); }

@@ -53,5 +53,9 @@ struct lbm_handle;
 namespace lbm {
 // Mesh size of a solver handle (library-internal accessor).
 void handle_dims(const lbm_handle *h, int *nx, int *ny, int *nz);
+// Warm start on the device: the equilibrium moments of a coarse S^3 field
+// (device memory, 4 channels u/U, v/U, w/U, p'/U^2 at stride cstride floats per
+// point) interpolated onto the handle's nodes, then its face buffers rebuilt.
+void warm_moments(lbm_handle *h, const float *field, int S, int cstride, double U);
 
 }  // namespace lbm

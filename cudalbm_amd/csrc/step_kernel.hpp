@@ -429,6 +429,11 @@ __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__res
                                                const ST *edge_m, int tid, int bx, int by, int z) {
     using E = Edge<TX, TY>;
     if (tid >= E::NE) return;
+    // the plane index is made opaque here: otherwise the compiler keeps every
+    // face slot's write offset as a loop induction variable in its own VGPR,
+    // advanced on every plane by every wave, although only the waves holding
+    // edge nodes ever store through them (18 VALU per plane and wave for D3Q19)
+    asm volatile("" : "+s"(z));
     int tx, ty;
     E::coords(tid, tx, ty);
     float m[10];

@@ -614,69 +614,16 @@ int lbm_sur_warm_start(lbm_sur *s, lbm_handle *h, double Re, double u_inf) {
         if (nx != ny || ny != nz) throw ArgError{LBM_ERR_ARG, "warm start needs a cubic mesh"};
         const int S = s->S, N = nx;
         const int64_t ps = s->points(0, 1);
-        std::vector<float> in((size_t)(ps * kInUsed)), f((size_t)(ps * kOutUsed));
+        std::vector<float> in((size_t)(ps * kInUsed));
         lbm_sur_encode(S, Re, N, in.data());
         // predict one sample (inlined to keep the error state on this object)
         ensure_batch(s, std::max(1, s->cap));
         upload_inputs(s, 1, in.data());
         forward(s, 1);
-        std::vector<float> o((size_t)(ps * 8));
-        HIPX(hipMemcpyAsync(o.data(), s->out, sizeof(float) * ps * 8, hipMemcpyDeviceToHost, s->stream));
         HIPX(hipStreamSynchronize(s->stream));
-        for (int64_t p = 0; p < ps; p++)
-            for (int c = 0; c < kOutUsed; c++) f[(size_t)(p * kOutUsed + c)] = o[(size_t)(p * 8 + c)];
-        // trilinear interpolation: network node i sits at solver coordinate i (N-1)/(S-1)
-        const int64_t n = (int64_t)N * N * N;
-        std::vector<std::vector<float>> mom(10, std::vector<float>((size_t)n));
-        std::vector<int> i0(N);
-        std::vector<float> fr(N);
-        for (int j = 0; j < N; j++) {
-            const double sc = N > 1 ? (double)j * (S - 1) / (N - 1) : 0.0;
-            int k = std::min(S - 2, (int)std::floor(sc));
-            i0[j] = k;
-            fr[j] = (float)(sc - k);
-        }
-        const double U = u_inf;
-        for (int z = 0; z < N; z++)
-            for (int y = 0; y < N; y++)
-                for (int x = 0; x < N; x++) {
-                    float val[kOutUsed];
-                    for (int c = 0; c < kOutUsed; c++) {
-                        float acc = 0.f;
-                        for (int d = 0; d < 8; d++) {
-                            const int ix = i0[x] + (d & 1), iy = i0[y] + ((d >> 1) & 1), iz = i0[z] + (d >> 2);
-                            const float wx = (d & 1) ? fr[x] : 1.f - fr[x];
-                            const float wy = ((d >> 1) & 1) ? fr[y] : 1.f - fr[y];
-                            const float wz = (d >> 2) ? fr[z] : 1.f - fr[z];
-                            acc += wx * wy * wz * f[(size_t)((((int64_t)iz * S + iy) * S + ix) * kOutUsed + c)];
-                        }
-                        val[c] = acc;
-                    }
-                    double u = U * val[0], v = U * val[1], w = U * val[2];
-                    const double rm1 = 3.0 * U * U * val[3];
-                    // walls at rest, lid moving +x (the boundary conditions own these nodes)
-                    if (y == N - 1) {
-                        u = U;
-                        v = w = 0.0;
-                    } else if (x == 0 || x == N - 1 || y == 0 || z == 0 || z == N - 1) {
-                        u = v = w = 0.0;
-                    }
-                    const int64_t q = ((int64_t)z * N + y) * N + x;
-                    mom[0][q] = (float)rm1;
-                    mom[1][q] = (float)(3.0 * u);
-                    mom[2][q] = (float)(3.0 * v);
-                    mom[3][q] = (float)(3.0 * w);
-                    mom[4][q] = (float)(4.5 * u * u);
-                    mom[5][q] = (float)(9.0 * u * v);
-                    mom[6][q] = (float)(9.0 * u * w);
-                    mom[7][q] = (float)(4.5 * v * v);
-                    mom[8][q] = (float)(9.0 * v * w);
-                    mom[9][q] = (float)(4.5 * w * w);
-                }
-        const float *mp[10];
-        for (int k = 0; k < 10; k++) mp[k] = mom[k].data();
-        const int rc = lbm_set_moments(h, mp, LBM_LAYOUT_GLOBAL);
-        if (rc != LBM_OK) throw ArgError{rc, std::string("lbm_set_moments: ") + lbm_last_error(h)};
+        // trilinear interpolation onto the solver's nodes and the equilibrium
+        // moments, on the device, straight into the solver's layout
+        lbm::warm_moments(h, s->out, S, 8, u_inf);
     });
 }
 
