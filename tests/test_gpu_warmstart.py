@@ -71,3 +71,30 @@ def test_broken_network_falls_back(native_lib):
     assert warm["fell_back"] == 1 and warm["warm"] == 0
     assert warm["steps"] == cold["steps"] + 100  # the rejected check costs one check interval
     assert rel(vel(mw), vel(mc)) <= 5e-4
+
+
+def test_warm_start_128_held_out(native_lib):
+    """Config-5 scale: a 128^3 cavity of the held-out evaluation draw (seed no
+    training set uses; case 11: Re 597, lid 0.055) solved to residual 1e-6 cold
+    and warm with the 64^3-grid model, the cold run then continued towards
+    residual 1e-8 (at most 200k more steps; in FP32 the residual settles near
+    5e-7, so the continuation runs its 200k steps) as the converged solution.
+    Bounds (measured on MI355X: time ratio 0.44, warm field 3.3e-5 from the
+    converged one): warm wall time <= 0.6 x cold, both stopped fields within
+    1e-4 rel-L2 of the converged solution."""
+    from cudalbm_amd import warmstart as W
+
+    model = MODELS[1]
+    if not os.path.exists(model):
+        pytest.skip("no trained model in the tree")
+    case = W.draw_cases(64, W.HELDOUT_SEED, (128, 192), re_range=(400.0, 2500.0))[11]
+    assert case[0] == 128
+    net, _ = W.load_model(model)
+    with net:
+        (r,) = W.evaluate(net, [case], converge_tol=1e-8, converge_steps=200000,
+                          log=lambda m: print(m, flush=True))
+    assert r["steady"] and r["warm"]["warm"] == 1 and r["warm"]["residual"] < 1e-6
+    assert r["converged"]["residual"] < r["cold"]["residual"]
+    assert r["time_ratio"] <= 0.6
+    assert r["warm_vs_converged_rel_l2"] <= 1e-4
+    assert r["cold_vs_converged_rel_l2"] <= 1e-4
