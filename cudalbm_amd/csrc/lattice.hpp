@@ -139,14 +139,21 @@ __device__ __forceinline__ Rec rec_setup(const float (&m)[10]) {
 // The even part (diagonal, then off-diagonal terms) and the odd part are formed
 // in one fixed order with a canonical sign, so a pair q / opposite(q) and all
 // populations of one plane share them (common subexpressions).
+__host__ __device__ constexpr int opposite(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
+// Sign of the first nonzero component of c_q: H_q = W (even + odd_sign * odd).
+template <int q>
+__host__ __device__ constexpr int odd_sign() {
+    return cxq(q) != 0 ? cxq(q) : (cyq(q) != 0 ? cyq(q) : (czq(q) != 0 ? czq(q) : 1));
+}
+// Even part: a + the diagonal terms, then the off-diagonal sum with its
+// canonical sign (first nonzero product positive); equal for q and opposite(q).
 template <int Q, int q>
-__device__ __forceinline__ float rec_dev(const Rec &r) {
+__device__ __forceinline__ float rec_even(const Rec &r) {
     constexpr int x = cxq(q), y = cyq(q), z = czq(q);
     float e = r.a;
     if constexpr (x != 0) e += r.M[4];
     if constexpr (y != 0) e += r.M[7];
     if constexpr (z != 0) e += r.M[9];
-    // off-diagonal part, canonical sign: first nonzero product positive
     constexpr int pxy = x * y, pxz = x * z, pyz = y * z;
     constexpr int s_off = pxy != 0 ? pxy : (pxz != 0 ? pxz : (pyz != 0 ? pyz : 1));
     if constexpr (pxy != 0 || pxz != 0 || pyz != 0) {
@@ -157,17 +164,42 @@ __device__ __forceinline__ float rec_dev(const Rec &r) {
         if constexpr (pyz != 0) { od = first ? r.M[8] : (pyz * s_off > 0 ? od + r.M[8] : od - r.M[8]); }
         e = s_off > 0 ? e + od : e - od;
     }
-    // odd part, canonical sign
-    constexpr int s_odd = x != 0 ? x : (y != 0 ? y : (z != 0 ? z : 1));
+    return e;
+}
+// Odd part with the canonical sign (q != 0); equal for q and opposite(q).
+template <int q>
+__device__ __forceinline__ float rec_odd(const Rec &r) {
+    constexpr int x = cxq(q), y = cyq(q), z = czq(q);
+    constexpr int s_odd = odd_sign<q>();
+    float o = 0.f;
+    bool first = true;
+    if constexpr (x != 0) { o = r.M[1]; first = false; }
+    if constexpr (y != 0) { o = first ? r.M[2] : (y * s_odd > 0 ? o + r.M[2] : o - r.M[2]); first = false; }
+    if constexpr (z != 0) { o = first ? r.M[3] : (z * s_odd > 0 ? o + r.M[3] : o - r.M[3]); }
+    return o;
+}
+template <int Q, int q>
+__device__ __forceinline__ float rec_dev(const Rec &r) {
+    const float e = rec_even<Q, q>(r);
     float h = e;
-    if constexpr (x != 0 || y != 0 || z != 0) {
-        float o = 0.f;
-        bool first = true;
-        if constexpr (x != 0) { o = r.M[1]; first = false; }
-        if constexpr (y != 0) { o = first ? r.M[2] : (y * s_odd > 0 ? o + r.M[2] : o - r.M[2]); first = false; }
-        if constexpr (z != 0) { o = first ? r.M[3] : (z * s_odd > 0 ? o + r.M[3] : o - r.M[3]); }
-        h = s_odd > 0 ? e + o : e - o;
+    if constexpr (q != 0) {
+        const float o = rec_odd<q>(r);
+        h = odd_sign<q>() > 0 ? e + o : e - o;
     }
+    constexpr int W = wint<Q>(q);
+    if constexpr (W == 1)
+        return h;
+    else
+        return float(W) * h;
+}
+// H_q for sgn = +1 and H_opposite(q) for sgn = -1 (a per-lane choice between the
+// two populations of a pair), with the same bits as rec_dev: e +/- o rounded once.
+template <int Q, int q>
+__device__ __forceinline__ float rec_signed(const Rec &r, float sgn) {
+    static_assert(q != 0, "the rest population has no pair");
+    const float e = rec_even<Q, q>(r);
+    const float o = rec_odd<q>(r);
+    const float h = odd_sign<q>() > 0 ? fmaf(sgn, o, e) : fmaf(-sgn, o, e);
     constexpr int W = wint<Q>(q);
     if constexpr (W == 1)
         return h;
@@ -181,25 +213,42 @@ struct Sums {
     float s0, sx, sy, sz, sxx, sxy, sxz, syy, syz, szz;
 };
 
+// First pair (q, q+1), q odd, whose population q has the given component
+// pattern: sums start from their first term, so no accumulator is seeded with 0
+// (0 + x is not folded under IEEE signed-zero rules: one instruction per sum).
+template <int Q>
+__host__ __device__ constexpr int first_pair(int a, int b) {
+    for (int q = 1; q < Q; q += 2)
+        if (cq(q, a) != 0 && (b < 0 || cq(q, b) != 0)) return q;
+    return -1;
+}
 template <int Q>
 __device__ __forceinline__ Sums pop_sums(const float (&g)[27]) {
-    Sums s{g[0], 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    Sums s;
+    s.s0 = g[0];
     unroll<0, (Q - 1) / 2>([&](auto pc) {
         constexpr int q = 1 + 2 * decltype(pc)::value;  // q and q+1 are opposite
         constexpr int x = cxq(q), y = cyq(q), z = czq(q);
         static_assert(cxq(q + 1) == -x && cyq(q + 1) == -y && czq(q + 1) == -z, "pair order");
         const float P = g[q] + g[q + 1];
         const float Mi = g[q] - g[q + 1];
+        // acc: s += sign * v, or s = sign * v for the sum's first term
+        auto acc = [](float &sum, float v, bool first, bool pos) {
+            if (first)
+                sum = pos ? v : -v;
+            else
+                sum = pos ? sum + v : sum - v;
+        };
         s.s0 += P;
-        if constexpr (x != 0) s.sx = x > 0 ? s.sx + Mi : s.sx - Mi;
-        if constexpr (y != 0) s.sy = y > 0 ? s.sy + Mi : s.sy - Mi;
-        if constexpr (z != 0) s.sz = z > 0 ? s.sz + Mi : s.sz - Mi;
-        if constexpr (x != 0) s.sxx += P;
-        if constexpr (y != 0) s.syy += P;
-        if constexpr (z != 0) s.szz += P;
-        if constexpr (x * y != 0) s.sxy = x * y > 0 ? s.sxy + P : s.sxy - P;
-        if constexpr (x * z != 0) s.sxz = x * z > 0 ? s.sxz + P : s.sxz - P;
-        if constexpr (y * z != 0) s.syz = y * z > 0 ? s.syz + P : s.syz - P;
+        if constexpr (x != 0) acc(s.sx, Mi, q == first_pair<Q>(0, -1), x > 0);
+        if constexpr (y != 0) acc(s.sy, Mi, q == first_pair<Q>(1, -1), y > 0);
+        if constexpr (z != 0) acc(s.sz, Mi, q == first_pair<Q>(2, -1), z > 0);
+        if constexpr (x != 0) acc(s.sxx, P, q == first_pair<Q>(0, -1), true);
+        if constexpr (y != 0) acc(s.syy, P, q == first_pair<Q>(1, -1), true);
+        if constexpr (z != 0) acc(s.szz, P, q == first_pair<Q>(2, -1), true);
+        if constexpr (x * y != 0) acc(s.sxy, P, q == first_pair<Q>(0, 1), x * y > 0);
+        if constexpr (x * z != 0) acc(s.sxz, P, q == first_pair<Q>(0, 2), x * z > 0);
+        if constexpr (y * z != 0) acc(s.syz, P, q == first_pair<Q>(1, 2), y * z > 0);
     });
     return s;
 }
@@ -226,15 +275,16 @@ __device__ __forceinline__ float recip(float r) {
 template <int Q>
 __device__ __forceinline__ void interior_moments(const Sums &s, float (&m)[10]) {
     const float inv = recip(d3<Q>() + s.s0);
+    const float inv3 = 3.0f * inv, inv9 = 9.0f * inv;
     m[0] = div_d3<Q>(s.s0);
-    m[1] = 3.0f * s.sx * inv;
-    m[2] = 3.0f * s.sy * inv;
-    m[3] = 3.0f * s.sz * inv;
+    m[1] = s.sx * inv3;
+    m[2] = s.sy * inv3;
+    m[3] = s.sz * inv3;
     m[4] = fmaf(4.5f, s.sxx, -1.5f * s.s0) * inv;
-    m[5] = 9.0f * s.sxy * inv;
-    m[6] = 9.0f * s.sxz * inv;
+    m[5] = s.sxy * inv9;
+    m[6] = s.sxz * inv9;
     m[7] = fmaf(4.5f, s.syy, -1.5f * s.s0) * inv;
-    m[8] = 9.0f * s.syz * inv;
+    m[8] = s.syz * inv9;
     m[9] = fmaf(4.5f, s.szz, -1.5f * s.s0) * inv;
 }
 

@@ -282,7 +282,7 @@ struct lbm_handle {
     bool own_stream = false;
     void *mom[10] = {};    // the ten moment arrays, inside one allocation (mom_base)
     void *mom_base = nullptr;
-    int mplane = 0;        // elements from a plane block of a moment to its next (StepParams::mplane)
+    int mplane = 0;        // node words from a plane block of a moment to its next (StepParams::mplane)
     float *faces[2] = {nullptr, nullptr};
     int64_t fwords = 0, yoff = 0, zoff = 0;
     int parity = 0;
@@ -572,23 +572,32 @@ extern "C" {
 const char *lbm_create_error(void) { return g_create_error.c_str(); }
 
 // Layout of the ten moment arrays of a tiling with tx ty nodes per plane
-// block, in one allocation of `bytes`: interleaved per plane block (mplane = 10
-// tx ty) when every byte offset from mom[0] fits in 32 bits, else ten separate
+// block, in one allocation of `bytes`.  The arrays are made of 4-byte node
+// words: one FP32 moment each (10 arrays), or with FP16 storage the codes of
+// moments 2j and 2j+1 of a node in one word (5 arrays; moment 2j+1 two bytes
+// after moment 2j).  Interleaved per plane block (mplane = arrays x tx ty node
+// words) when every byte offset from mom[0] fits in 32 bits, else separate
 // arrays back to back (mplane = tx ty, 256-byte aligned).  Interleaving makes
 // each workgroup plane of all ten moments one contiguous run: a bare stream in
 // the step's access shape measured 5.60 TB/s against 5.45 TB/s for ten arrays.
+// kstride: bytes from one word array to the next.
 static void moment_layout(const lbm_handle *h, int tx, int ty, int *mplane, size_t *kstride, size_t *bytes) {
-    const size_t e = h->elem(), nt = (size_t)tx * ty, total = e * (size_t)h->n * 10;
+    const size_t nt = (size_t)tx * ty, arrays = h->elem() == 4 ? 10 : 5, total = 4 * (size_t)h->n * arrays;
     if (total <= ((size_t)1 << 32)) {
-        *mplane = (int)(10 * nt);
-        *kstride = e * nt;  // mom[k] = base + k tx ty elements
+        *mplane = (int)(arrays * nt);
+        *kstride = 4 * nt;  // word array a = base + a tx ty words
         *bytes = total;
     } else {
-        const size_t arr = (e * (size_t)h->n + 255) / 256 * 256;
+        const size_t arr = (4 * (size_t)h->n + 255) / 256 * 256;
         *mplane = (int)nt;
         *kstride = arr;
-        *bytes = arr * 10;
+        *bytes = arr * arrays;
     }
+}
+// Address of moment k in a layout of moment_layout.
+static void *moment_ptr(const lbm_handle *h, void *base, size_t kstride, int k) {
+    return h->elem() == 4 ? static_cast<char *>(base) + kstride * k
+                          : static_cast<char *>(base) + kstride * (k / 2) + 2 * (k % 2);
 }
 
 // Allocates the moments (at rest) for the handle's current tiling.
@@ -605,7 +614,7 @@ static void alloc_moments(lbm_handle *h) {
     }
     h->mom_base = base;
     h->mplane = mplane;
-    for (int i = 0; i < 10; i++) h->mom[i] = static_cast<char *>(base) + kstride * i;
+    for (int i = 0; i < 10; i++) h->mom[i] = moment_ptr(h, base, kstride, i);
 }
 
 // Moves the case to another automatic tiling (footprint tx x ty, lz planes):
@@ -650,14 +659,12 @@ static bool retile(lbm_handle *h, int tx, int ty, int lz, unsigned fo_mask) {
     const ColumnTiling from_plain{h->TX, h->TY, h->nbx, h->nby, h->TX * h->TY},
         to_plain{tx, ty, c.nx / tx, c.ny / ty, tx * ty};
     try {
-        for (int k = 0; k < 10; k++) {
-            void *dst = static_cast<char *>(base) + kstride * k;
-            if (h->elem() == 4)
-                hipLaunchKernelGGL(retile_kernel<float>, dim3(4096), dim3(256), 0, h->stream,
-                                   (const float *)h->mom[k], (float *)dst, h->n, from, to);
-            else
-                hipLaunchKernelGGL(retile_kernel<_Float16>, dim3(4096), dim3(256), 0, h->stream,
-                                   (const _Float16 *)h->mom[k], (_Float16 *)dst, h->n, from, to);
+        // node words: ten FP32 arrays, or five FP16 pair arrays (moments 2j, 2j+1)
+        const int step = h->elem() == 4 ? 1 : 2;
+        for (int k = 0; k < 10; k += step) {
+            hipLaunchKernelGGL(retile_kernel<uint32_t>, dim3(4096), dim3(256), 0, h->stream,
+                               (const uint32_t *)h->mom[k], (uint32_t *)moment_ptr(h, base, kstride, k), h->n, from,
+                               to);
             HIPX(hipGetLastError());
         }
         for (int a = 0; uprev && a < 3; a++) {
@@ -677,7 +684,7 @@ static bool retile(lbm_handle *h, int tx, int ty, int lz, unsigned fo_mask) {
     if (h->uprev) (void)hipFree(h->uprev);
     h->mom_base = base;
     h->mplane = mplane;
-    for (int k = 0; k < 10; k++) h->mom[k] = static_cast<char *>(base) + kstride * k;
+    for (int k = 0; k < 10; k++) h->mom[k] = moment_ptr(h, base, kstride, k);
     h->faces[0] = faces[0];
     h->faces[1] = faces[1];
     if (uprev) h->uprev = uprev;

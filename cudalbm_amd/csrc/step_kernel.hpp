@@ -56,12 +56,14 @@ struct StepClock {
 struct StepParams {
     int nx, ny, nz, lz, nbx, nby, nc;
     float omega, t_omega, omega_h, u_lid, tau;
-    // Stored moments, column-tiled internal layout, in place.  Node i of the
-    // internal order (plane block b = i / (tx ty), r = i mod (tx ty)) of moment k
-    // is element b * mplane + r of mom[k]: mplane = tx ty for ten separate
-    // arrays, 10 tx ty when the ten arrays are interleaved per plane block
-    // (mom[k] = base + k tx ty elements; every plane block of a workgroup then
-    // is one contiguous run of all ten moments).  mshift = log2(tx ty).
+    // Stored moments, column-tiled internal layout, in place, in 4-byte node
+    // words (one FP32 moment, or the FP16 codes of moments 2j and 2j+1, moment
+    // 2j+1 at mom[2j] + 2 bytes).  Node i of the internal order (plane block
+    // b = i / (tx ty), r = i mod (tx ty)) of moment k is in word b * mplane + r
+    // from mom[k]: mplane = tx ty for separate word arrays, (number of word
+    // arrays) tx ty when they are interleaved per plane block (every plane
+    // block of a workgroup then is one contiguous run of all ten moments).
+    // mshift = log2(tx ty).
     void *mom[10];
     int mplane, mshift;
     const float *fin;      // face buffers read (previous step): X | Y | Z
@@ -124,10 +126,16 @@ struct Store<float> {
     __device__ static __forceinline__ void store(void *p, unsigned off, float v) {
         __builtin_nontemporal_store(v, static_cast<float *>(byte_at(p, off)));
     }
+    __device__ static __forceinline__ void store_word(void *p, unsigned off, uint32_t w) {
+        __builtin_nontemporal_store(w, static_cast<uint32_t *>(byte_at(p, off)));
+    }
 };
+// FP16: the codes of moments 2j (low half) and 2j + 1 (high half) of a node
+// share one 32-bit node word (mom[2j + 1] = mom[2j] + 2 bytes), so a plane of
+// ten codes is five 4-byte loads and stores per node; kBytes is the node stride.
 template <>
 struct Store<_Float16> {
-    static constexpr unsigned kBytes = 2;
+    static constexpr unsigned kBytes = 4;
     __device__ static __forceinline__ float load(const void *p, unsigned off, float dec) {
         return float(__builtin_nontemporal_load(static_cast<const _Float16 *>(byte_at(p, off)))) * dec;
     }
@@ -143,8 +151,18 @@ struct Store<_Float16> {
     __device__ static __forceinline__ float from_raw(float w) {
         return (float)__builtin_bit_cast(_Float16, (uint16_t)__float_as_uint(w));
     }
+    __device__ static __forceinline__ float from_raw_hi(float w) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)(__float_as_uint(w) >> 16));
+    }
     __device__ static __forceinline__ void store(void *p, unsigned off, _Float16 v) {
         __builtin_nontemporal_store(v, static_cast<_Float16 *>(byte_at(p, off)));
+    }
+    // the node word of moments 2j, 2j + 1 (p = mom[2j]), as float bits
+    __device__ static __forceinline__ float load_word(const void *p, unsigned off) {
+        return __uint_as_float(__builtin_nontemporal_load(static_cast<const uint32_t *>(byte_at(p, off))));
+    }
+    __device__ static __forceinline__ void store_word(void *p, unsigned off, uint32_t w) {
+        __builtin_nontemporal_store(w, static_cast<uint32_t *>(byte_at(p, off)));
     }
 };
 
@@ -210,6 +228,24 @@ struct Job {
     __device__ __forceinline__ int dz() const { return (zsd & 3) - 1; }
 };
 
+// Offsets of one population inside the face blocks.  X faces [bx][slot][z][side][y],
+// Y faces [by][slot][z][side][x]: the two sides of a column's face (W/E, S/N) sit
+// next to each other, so a face row's lanes of both sides differ only in a
+// small lane offset (side * ny + y) from a base that is uniform in the workgroup.
+// Z faces [side][chunk][slot][y][x].
+// All offsets are 32-bit: the host rejects meshes whose arrays exceed 2^31 words.
+template <int Q>
+__host__ __device__ __forceinline__ int xf_off(const StepParams &p, int s, int bx, int k, int z, int y) {
+    return (((bx * VS<Q>::QF + k) * p.nz + z) * 2 + s) * p.ny + y;
+}
+template <int Q>
+__host__ __device__ __forceinline__ int yf_off(const StepParams &p, int s, int by, int k, int z, int x) {
+    return p.yoff + ((((by * VS<Q>::QF + k) * p.nz + z) * 2 + s) * p.nx + x);
+}
+template <int Q>
+__device__ __forceinline__ int zf_off(const StepParams &p, int s, int c, int k, int y, int x) {
+    return p.zoff + (((s * p.nc + c) * VS<Q>::QF + k) * p.ny + y) * p.nx + x;
+}
 // Ring job j of this workgroup: (side, face slot k, position) decoded against the
 // compile-time segment table, so the population index is a constant per branch.
 template <int Q, int TX, int TY>
@@ -246,13 +282,13 @@ __device__ __forceinline__ Job make_job(int j, const StepParams &p, int bx, int 
                 if constexpr (side < 2) {
                     const int col = side == 0 ? bx - 1 : bx + 1;
                     constexpr int s_face = side == 0 ? 1 : 0;  // east (1) / west (0) faces
-                    job.zsd = p.ny * 4 + (dzq + 1);
-                    job.gb = inside ? (((s_face * p.nbx + col) * QF + k) * p.nz) * p.ny + gy : -1;
+                    job.zsd = 2 * p.ny * 4 + (dzq + 1);
+                    job.gb = inside ? xf_off<Q>(p, s_face, col, k, 0, gy) : -1;
                 } else {
                     const int row = side == 2 ? by - 1 : by + 1;
                     constexpr int s_face = side == 2 ? 1 : 0;
-                    job.zsd = p.nx * 4 + (dzq + 1);
-                    job.gb = inside ? p.yoff + (((s_face * p.nby + row) * QF + k) * p.nz) * p.nx + gx : -1;
+                    job.zsd = 2 * p.nx * 4 + (dzq + 1);
+                    job.gb = inside ? yf_off<Q>(p, s_face, row, k, 0, gx) : -1;
                 }
             }
             start += cnt;
@@ -274,20 +310,6 @@ __device__ __forceinline__ float job_value(const Job &j, float raw, int zdest, i
     return (j.gb >= 0 && zs >= 0 && zs < nz) ? raw : 0.f;
 }
 
-// Offsets of one population inside the face blocks.
-// All offsets are 32-bit: the host rejects meshes whose arrays exceed 2^31 words.
-template <int Q>
-__device__ __forceinline__ int xf_off(const StepParams &p, int s, int bx, int k, int z, int y) {
-    return (((s * p.nbx + bx) * VS<Q>::QF + k) * p.nz + z) * p.ny + y;
-}
-template <int Q>
-__device__ __forceinline__ int yf_off(const StepParams &p, int s, int by, int k, int z, int x) {
-    return p.yoff + (((s * p.nby + by) * VS<Q>::QF + k) * p.nz + z) * p.nx + x;
-}
-template <int Q>
-__device__ __forceinline__ int zf_off(const StepParams &p, int s, int c, int k, int y, int x) {
-    return p.zoff + (((s * p.nc + c) * VS<Q>::QF + k) * p.ny + y) * p.nx + x;
-}
 // Population q of node (x,y,z) from its rebuild terms (or its probe id).
 template <int Q, int q, bool PROBE>
 __device__ __forceinline__ float pop_of(const Rec &r, int x, int y, int z,
@@ -383,7 +405,8 @@ struct Edge {
 };
 
 // Population-rebuild terms from moments as stored: the raw words of
-// load_plane_raw / Store::raw_word.  FP32: rec_setup.  FP16: decoded by
+// load_plane_raw.  FP32: rec_setup.  FP16: h[j] (j < 5) holds node word j
+// (moment 2j in its low half, 2j + 1 in its high half), decoded by
 // mixed-precision FMAs, FP16 operand taken directly (no separate conversion);
 // the decode factors are equal within the groups {1,2,3}, {4,7,9}, {5,6,8} (the
 // codec's u, Pi_aa and Pi_ab scales) and the density code is 3 (rho - 1) itself.
@@ -394,7 +417,7 @@ __device__ __forceinline__ Rec rec_setup_st(const float (&h)[10]) {
     if constexpr (sizeof(ST) == 4) {
         return rec_setup(h);
     } else {
-        auto hv = [&](int k) { return Store<ST>::from_raw(h[k]); };
+        auto hv = [&](int k) { return (k & 1) ? Store<ST>::from_raw_hi(h[k / 2]) : Store<ST>::from_raw(h[k / 2]); };
         Rec r;
         const float rho = fmaf(hv(0), codec_dec(0), 1.0f);
         float sd = fmaf(hv(9), codec_dec(4), 0.0f);
@@ -417,34 +440,73 @@ __device__ __forceinline__ Rec rec_setup_st(const float (&h)[10]) {
     }
 }
 
+// Edge-node moments as stored, parked in LDS for the deferred face flush
+// (park: [10][NE] FP32 values, or [5][NE] FP16 node words) and read back in
+// the raw-word form of load_plane_raw.
+template <typename ST, int NE>
+__device__ __forceinline__ void park_read(const ST *park, int e, float (&m)[10]) {
+    if constexpr (sizeof(ST) == 4) {
+#pragma unroll
+        for (int k = 0; k < 10; k++) m[k] = park[k * NE + e];
+    } else {
+        const uint32_t *pw = reinterpret_cast<const uint32_t *>(park);
+#pragma unroll
+        for (int j = 0; j < 5; j++) m[j] = __uint_as_float(pw[j * NE + e]);
+    }
+}
+
 // x/y face populations of one finished plane, written cooperatively: edge
 // nodes park their post-collision moments in LDS (edge_m, 10 x NE values as
-// stored: FP32, or the FP16 codes that rec_setup_st decodes) and
+// stored: FP32, or 5 x NE FP16 node words that rec_setup_st decodes) and
 // the first NE threads -- whole waves where TX is a multiple of 32 -- rebuild
 // and store the face populations, instead of every wave rebuilding them for
 // its few edge lanes.  Row entries write the y faces, column entries the x
-// faces, so a wave of row entries runs only the y-face branches.
+// faces.  The two faces of one kind carry opposite populations (slot k of the
+// +axis face holds q, slot k of the -axis face opposite(q)), which share the
+// even part of the rebuild: a lane forms its face's population with its own
+// sign (rec_signed), so the S and N rows (W and E columns) run one instruction
+// stream, and the store address is a uniform base per slot plus a 32-bit lane
+// offset.
 template <int Q, int TX, int TY, bool PROBE, typename ST>
 __device__ __forceinline__ void flush_xy_faces(const StepParams &p, float *__restrict__ fout,
                                                const ST *edge_m, int tid, int bx, int by, int z) {
     using E = Edge<TX, TY>;
+    constexpr int QF = VS<Q>::QF;
     if (tid >= E::NE) return;
     // the plane index is made opaque here: otherwise the compiler keeps every
     // face slot's write offset as a loop induction variable in its own VGPR,
     // advanced on every plane by every wave, although only the waves holding
     // edge nodes ever store through them (18 VALU per plane and wave for D3Q19)
     asm volatile("" : "+s"(z));
+    const bool row = tid < 2 * TX;
+    // side 0: the S row / W column (populations with c = -1), side 1: N / E (c = +1)
+    const int side = row ? (tid >= TX ? 1 : 0) : (tid >= 2 * TX + TY ? 1 : 0);
+    const bool neighbour = row ? (side ? by < p.nby - 1 : by > 0) : (side ? bx < p.nbx - 1 : bx > 0);
+    if (!neighbour) return;
     int tx, ty;
     E::coords(tid, tx, ty);
     float m[10];
-#pragma unroll
-    for (int k = 0; k < 10; k++) m[k] = Store<ST>::raw_word(edge_m[k * E::NE + tid]);
+    park_read<ST, E::NE>(edge_m, tid, m);
     const Rec r = rec_setup_st<ST>(m);
+    const float sgn = side ? 1.f : -1.f;
     const int x = bx * TX + tx, y = by * TY + ty;
-    if (tid < 2 * TX)
-        write_y_faces<Q, TX, TY, PROBE>(p, fout, r, ty, x, y, z, by);
+    auto put = [&](auto kc, auto axc) {
+        constexpr int k = decltype(kc)::value, axis = decltype(axc)::value;
+        constexpr int q = face_q<Q>(axis, 1, k);
+        static_assert(face_q<Q>(axis, -1, k) == opposite(q), "face slots pair opposite populations");
+        float v;
+        if constexpr (PROBE)
+            v = probe_id(x, y, z, p.nx, p.ny, side ? q : opposite(q));
+        else
+            v = rec_signed<Q, q>(r, sgn);
+        float *base = fout + (axis == 0 ? xf_off<Q>(p, 0, bx, k, z, 0) : yf_off<Q>(p, 0, by, k, z, 0));
+        const unsigned off = (unsigned)(axis == 0 ? side * p.ny + y : side * p.nx + x) * 4u;
+        *static_cast<float *>(byte_at(base, off)) = v;
+    };
+    if (row)
+        unroll<0, QF>([&](auto kc) { put(kc, std::integral_constant<int, 1>{}); });
     else
-        write_x_faces<Q, TX, TY, PROBE>(p, fout, r, tx, x, y, z, bx);
+        unroll<0, QF>([&](auto kc) { put(kc, std::integral_constant<int, 0>{}); });
 }
 
 // Plane block of this workgroup's tile in plane z (uniform).
@@ -469,35 +531,26 @@ __device__ __forceinline__ unsigned mom_off(const StepParams &p, int64_t i) {
     return ((u >> p.mshift) * (unsigned)p.mplane + (u & ((1u << p.mshift) - 1u))) * Store<ST>::kBytes;
 }
 
-// The same plane as stored (FP16: not yet scaled by the codec's decode
-// factors, which rec_setup_st applies).
-template <typename ST, int TX, int TY>
+// Raw words of one stored plane: 10 FP32 moments, or 5 FP16 node words.
+template <typename ST>
+constexpr int kRawWords = sizeof(ST) == 4 ? 10 : 5;
+
+// The same plane as stored, in m[0 .. kRawWords) (FP16: the five node words,
+// not yet scaled by the codec's decode factors, which rec_setup_st applies).
+// PROBE: zeros.
+template <typename ST, int TX, int TY, bool PROBE = false>
 __device__ __forceinline__ void load_plane_raw(const StepParams &p, int bx, int by, int z, int tid,
                                                float (&m)[10]) {
     const unsigned off = mom_plane_off<ST>(p, bx, by, z) + (unsigned)tid * Store<ST>::kBytes;
+    if constexpr (PROBE) {
 #pragma unroll
-    for (int k = 0; k < 10; k++) m[k] = Store<ST>::load_raw(p.mom[k], off);
-}
-
-// Next plane's old moments as stored (load_plane_raw), or past the chunk's last
-// plane the down-going populations of the plane above it (bottom face of chunk
-// c+1; zero above the domain) in m[0..QF-1].
-template <int Q, typename ST, int TX, int TY, bool PROBE>
-__device__ __forceinline__ void load_next(const StepParams &p, const float *__restrict__ fin, int bx, int by,
-                                          int c, int zn, int z1, int tid, int x, int y, float (&m)[10]) {
-    if (zn <= z1) {
-        if constexpr (!PROBE) {
-            load_plane_raw<ST, TX, TY>(p, bx, by, zn, tid, m);
-        } else {
+        for (int k = 0; k < 10; k++) m[k] = 0.f;
+    } else if constexpr (sizeof(ST) == 4) {
 #pragma unroll
-            for (int k = 0; k < 10; k++) m[k] = 0.f;
-        }
+        for (int k = 0; k < 10; k++) m[k] = Store<ST>::load_raw(p.mom[k], off);
     } else {
 #pragma unroll
-        for (int k = 0; k < VS<Q>::QF; k++)
-            m[k] = c < p.nc - 1 ? fin[zf_off<Q>(p, 0, c + 1, k, y, x)] : 0.f;
-#pragma unroll
-        for (int k = VS<Q>::QF; k < 10; k++) m[k] = 0.f;
+        for (int j = 0; j < 5; j++) m[j] = Store<ST>::load_word(p.mom[2 * j], off);
     }
 }
 
@@ -517,11 +570,11 @@ __device__ __forceinline__ unsigned hash32(unsigned x) {
     x ^= x >> 16;
     return x;
 }
-__device__ __forceinline__ _Float16 sr_half(float v, unsigned r) {
-    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-    half2_t o = {(_Float16)0.f, (_Float16)0.f};
-    o = __builtin_amdgcn_cvt_sr_f16_f32(o, v, r, false);
-    return o[0];
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+// v rounded stochastically into half `hi` of w (the other half kept)
+template <bool HI>
+__device__ __forceinline__ half2_t sr_into(half2_t w, float v, unsigned r) {
+    return __builtin_amdgcn_cvt_sr_f16_f32(w, v, r, HI);
 }
 
 // Function-object values of one node from its stored (scaled) moments:
