@@ -59,8 +59,28 @@ struct VS {
     static constexpr int NU = Q == 19 ? 4 : 8;  // up-going pops that also move in-plane
 };
 
-// LDS slot of a population that moves in-plane (q not in {0,5,6}).
-__host__ __device__ constexpr int lds_slot(int q) { return q <= 4 ? q - 1 : q - 3; }
+// Up-going populations that also move in-plane (c_z = +1, q != 5), and the
+// index of one among them (ascending q).
+__host__ __device__ constexpr bool up_moving(int q) { return czq(q) == 1 && (cxq(q) != 0 || cyq(q) != 0); }
+template <int Q>
+__host__ __device__ constexpr int u_index(int q) {
+    int n = 0;
+    for (int p = 1; p < q; p++)
+        if (up_moving(p)) n++;
+    return n;
+}
+// LDS slot of a population that moves in-plane (q not in {0,5,6}): the others
+// first in ascending q, the NU up-going ones last in u_index order, so that
+// their second bank (slots NSLOT + u_index) is the first bank shifted by NU
+// slots for every one of them.
+template <int Q>
+__host__ __device__ constexpr int lds_slot(int q) {
+    if (up_moving(q)) return VS<Q>::NSLOT - VS<Q>::NU + u_index<Q>(q);
+    int n = 0;
+    for (int p = 1; p < q; p++)
+        if (p != 5 && p != 6 && !up_moving(p)) n++;
+    return n;
+}
 
 // k-th population (ascending q) with c[axis] == sign; the reference face-slot
 // order (halo.cuh:813-904).

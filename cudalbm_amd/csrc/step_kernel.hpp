@@ -202,19 +202,14 @@ struct Ring {
     static constexpr int JPT = (NJ + TX * TY - 1) / (TX * TY);
 };
 
-// Index of an up-going, in-plane-moving population among its kind.
-template <int Q>
-__host__ __device__ constexpr int u_index(int q) {
-    int n = 0;
-    for (int p = 1; p < q; p++)
-        if (czq(p) == 1 && (cxq(p) != 0 || cyq(p) != 0)) n++;
-    return n;
-}
-// LDS base of the slot holding up-going population q in bank 0 or 1.
+// LDS base of the slot holding up-going population q in bank 0 or 1 (bank 1 is
+// bank 0 shifted by NU slots, see lds_slot).
 template <int Q, int TX, int TY>
 __host__ __device__ constexpr int u_addr(int q, int bank) {
-    return bank ? (VS<Q>::NSLOT + u_index<Q>(q)) * Geo<TX, TY>::P : lds_slot(q) * Geo<TX, TY>::P;
+    return (lds_slot<Q>(q) + bank * VS<Q>::NU) * Geo<TX, TY>::P;
 }
+static_assert(lds_slot<19>(9) == 12 && lds_slot<19>(18) == 15 && lds_slot<19>(17) == 11, "D3Q19 LDS slots");
+static_assert(lds_slot<27>(9) == 16 && lds_slot<27>(25) == 23 && lds_slot<27>(26) == 15, "D3Q27 LDS slots");
 // Ring job in three registers: LDS word offsets of bank 0 / bank 1 (16 bits
 // each; `lds` = -1 when the thread has no job), the face-set offset without the
 // plane term (-1 when outside the domain), and plane stride * 4 + (dz + 1).
@@ -273,7 +268,7 @@ __device__ __forceinline__ Job make_job(int j, const StepParams &p, int bx, int 
                     rx = pos + (cxq(q) < 0 ? 1 : 0);  // rx in [max(0,-cx), min(TX-1, TX-1-cx)]
                 }
                 const int cell = (ry + 1) * G::WP + (rx + 1);
-                const int l0 = lds_slot(q) * G::P + cell;
+                const int l0 = lds_slot<Q>(q) * G::P + cell;
                 const int l1 = czq(q) == 1 ? u_addr<Q, TX, TY>(q, 1) + cell : l0;
                 job.lds = l0 | (l1 << 16);
                 constexpr int dzq = -czq(q);
@@ -576,7 +571,6 @@ template <bool HI>
 __device__ __forceinline__ half2_t sr_into(half2_t w, float v, unsigned r) {
     return __builtin_amdgcn_cvt_sr_f16_f32(w, v, r, HI);
 }
-
 // Function-object values of one node from its stored (scaled) moments:
 // k = kineticEnergy.cuh:71-85; S = strainRateTensor.cuh:71-84, i.e.
 // S_aa = 3(m_a^2 - m_aa)/(2 * 4.5 tau) = (m_a^2 - m_aa) s_diag and
