@@ -328,7 +328,10 @@ def main():
             r = measure(nv, Q, st, k, 10, fo_mask=fo)
             variants[name] = {"n": nv, "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
                               "roofline_frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4),
-                              "bytes_per_node": r["bytes_per_node"], "tiling": r["tiling"]}
+                              "bytes_per_node": r["bytes_per_node"], "tiling": r["tiling"],
+                              "traffic": measured_traffic(
+                                  f"step_kernel<{Q},{r['tiling'][0]},{r['tiling'][1]},"
+                                  f"{'_Float16' if st == 'fp16' else 'float'}{',fo' if fo else ''}>@{nv}^3")}
             print(f"# {name}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         variants["batch_sweep64_d3q19_fp32"] = batch_sweep(100)
         print(f"# batch sweep: {variants['batch_sweep64_d3q19_fp32']}", file=sys.stderr, flush=True)

@@ -124,6 +124,34 @@ def test_layout_tiled8_round_trip(native_lib):
         assert np.array_equal(back_t[k], tiled[k])
 
 
+@pytest.mark.parametrize("Q", [19, 27])
+def test_fp16_node_words_round_trip(native_lib, Q):
+    """FP16 storage keeps the codes of moments 2j and 2j+1 of a node in one
+    32-bit word (low / high half).  Moments whose scaled values are FP16 codes
+    (code * decode factor) come back exactly, in both layouts, and so does a
+    distinct random code per moment and node: a swapped half, a wrong word
+    stride or a wrong pair would move codes between moments or nodes."""
+    from cudalbm_amd import Solver
+    from cudalbm_amd import native as N
+
+    # decode factors of the codec: (rho-1)/cs2, u/cs, Pi_aa and Pi_ab scales
+    enc = np.array([3.0, *[1 / np.sqrt(3.0)] * 3, 2 / 3, 1 / 3, 1 / 3, 2 / 3, 1 / 3, 2 / 3])
+    dec = (1.0 / enc).astype(np.float32)
+    nx, ny, nz = 32, 16, 24
+    rng = np.random.default_rng(11)
+    codes = [rng.uniform(-0.5, 0.5, nx * ny * nz).astype(np.float16) for _ in range(10)]
+    want = [c.astype(np.float32) * dec[k] for k, c in enumerate(codes)]
+    with Solver(nx, ny, nz, 100.0, 0.05, velocity_set=Q, storage="fp16", tile=(16, 16, 4)) as s:
+        s.set_moments(want)
+        back = s.get_moments()
+        back_t = s.get_moments(layout=N.LBM_LAYOUT_TILED8)
+        s.set_moments(back_t, layout=N.LBM_LAYOUT_TILED8)
+        again = s.get_moments()
+    for k in range(10):
+        assert np.array_equal(back[k], want[k]), k
+        assert np.array_equal(again[k], want[k]), k
+
+
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 def test_fo_enable_keeps_the_moments(native_lib, storage):
     """Turning the FP64 means on re-tiles an automatically tiled D3Q19 case
