@@ -51,32 +51,35 @@ def measured_traffic(key):
         return None
 
 
-def warm_vs_cold(n_cases=3, edge=128, max_steps=200000, max_re=1000.0):
-    """The first n_cases edge^3 cavities with Re <= max_re of the held-out
-    evaluation draw (64 cavities, a seed no training set uses, edges {128,
-    192}, Re log-uniform [400, 2500]) solved to residual 1e-6 from rest and
-    from the trained surrogate's prediction.  The Re filter is fixed in advance:
-    over the whole draw no 128^3 cavity above Re 1117 reached a steady state in
-    200k steps, so higher-Re cases only time two capped runs.  Every selected
-    case is reported; one whose cold run does not reach 1e-6 in max_steps is
-    flagged and left out of the medians.  The whole 64-case evaluation is
-    `python -m cudalbm_amd.warmstart eval` (a run of many minutes)."""
+def warm_vs_cold(n_cases=3, n_cases_192=1, max_steps=200000, max_re=1000.0):
+    """The first n_cases 128^3 cavities and the first n_cases_192 192^3 cavities
+    with Re <= max_re of the held-out evaluation draw (64 cavities, a seed no
+    training set uses, edges {128, 192}, Re log-uniform [400, 2500]) solved to
+    residual 1e-6 from rest and from the trained surrogate's prediction.  The
+    Re filter is fixed in advance: over the whole draw no 128^3 cavity above Re
+    1117 reached a steady state in 200k steps, so higher-Re cases only time two
+    capped runs.  Every selected case is reported; one whose cold run does not
+    reach 1e-6 in max_steps is flagged and left out of the medians.  The whole
+    64-case evaluation is `python -m cudalbm_amd.warmstart eval` (a run of many
+    minutes)."""
     from cudalbm_amd import warmstart as W
 
     net, meta = W.load_model(MODEL_FILE)
     with net:
         draw = W.draw_cases(64, W.HELDOUT_SEED, (128, 192), re_range=(400.0, 2500.0))
-        cases = [c for c in draw if c[0] == edge and c[1] <= max_re][:n_cases]
+        cases = ([c for c in draw if c[0] == 128 and c[1] <= max_re][:n_cases] +
+                 [c for c in draw if c[0] == 192 and c[1] <= max_re][:n_cases_192])
         res = W.evaluate(net, cases, max_steps=max_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
+    by_edge = {f"summary_{e}": W.summarise([r for r in res if r["N"] == e]) for e in sorted({r["N"] for r in res})}
     return {"cases": [{"N": r["N"], "Re": round(r["Re"], 1), "U": round(r["U"], 4), "steady": r["steady"],
                        "cold_steps": r["cold"]["steps"], "cold_ms": round(r["cold"]["ms"], 1),
                        "cold_residual": r["cold"]["residual"], "warm_steps": r["warm"]["steps"],
                        "warm_ms": round(r["warm"]["ms"], 1), "warm_residual": r["warm"]["residual"],
                        "warm_kept": bool(r["warm"]["warm"]), "time_ratio": round(r["time_ratio"], 4),
                        "final_velocity_rel_l2": r["final_rel_l2"]} for r in res],
-            **W.summarise(res),
+            **W.summarise(res), **by_edge,
             "model": {"file": os.path.basename(MODEL_FILE),
-                      **{k: meta[k] for k in ("S", "cases", "epochs", "final_loss") if k in meta}}}
+                      **{k: meta[k] for k in ("S", "cases", "epochs", "final_loss", "edges") if k in meta}}}
 
 
 def measure(n, Q, storage, steps, warmup, fo_mask=0, Re=1000.0, u=0.05, tile=(0, 0, 0)):

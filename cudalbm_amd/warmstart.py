@@ -4,9 +4,11 @@
     python -m cudalbm_amd.warmstart gen --set hi --out hi.npz      (data only)
     python -m cudalbm_amd.warmstart train --data base.npz hi.npz --out model.npz
     python -m cudalbm_amd.warmstart eval  --model cudalbm_amd/models/unet32.npz
+    python -m cudalbm_amd.warmstart bound --indices 6,25   (perfect-network bound)
 
 Data: lid-driven cavities (D3Q19 FP32) drawn like the batched sweep (Re
-log-uniform, lid speed uniform) on edges N in {32, 48, 64, 96, 128}, solved together by
+log-uniform, lid speed uniform) on edges N in {32, 48, 64, 96, 128} (and 128/192
+for the "x192" set), solved together by
 the batched solver (lbm_batch_run) until the residual
 ||u_t - u_{t-100}|| / ||u_t|| < 1e-6 or a step cap, then resampled onto the
 network's 32^3 grid (network node i at solver coordinate i (N-1)/31) and
@@ -35,7 +37,7 @@ from . import native as N
 TRAIN_SEED = 20261203
 EVAL_SEED = 20261204
 # held-out evaluation draws: a seed no training set uses (the training sets use
-# TRAIN_SEED and TRAIN_SEED + 1, which equals EVAL_SEED)
+# TRAIN_SEED, TRAIN_SEED + 1 -- which equals EVAL_SEED -- and X192_SEED)
 HELDOUT_SEED = 20261207
 S_NET = 32
 # training edges: the sweep's range up to 128, weighted towards the cheap ones
@@ -43,9 +45,15 @@ TRAIN_EDGES = (32, 48, 64, 96, 128)
 TRAIN_EDGE_P = (0.25, 0.25, 0.25, 0.15, 0.10)
 # data sets: "base" as above; "hi" adds cavities at the large edges, where a
 # 32^3 prediction is interpolated furthest, with a higher step cap
+# "x192": cavities at the evaluation edges 128 and 192 (192 lies beyond the
+# other sets' edges) in the Re range where those meshes reach a steady state,
+# drawn with a seed that no other set and no evaluation draw uses
+X192_SEED = 20261211
 DATA_SETS = {
     "base": {"cases": 512, "seed": TRAIN_SEED, "edges": TRAIN_EDGES, "edge_p": TRAIN_EDGE_P, "max_steps": 80000},
     "hi": {"cases": 160, "seed": TRAIN_SEED + 1, "edges": (96, 128), "edge_p": (0.4, 0.6), "max_steps": 150000},
+    "x192": {"cases": 96, "seed": X192_SEED, "edges": (128, 192), "edge_p": (0.5, 0.5), "max_steps": 150000,
+             "re_range": (200.0, 1400.0)},
 }
 
 
@@ -197,6 +205,62 @@ def evaluate(net, cases, tol=1e-6, check=100, max_steps=200000, log=print, conve
     return out
 
 
+def upsample(f4, S, n):
+    """[S^3][4] network-grid fields -> n^3 x 4 (trilinear, network node i at
+    solver coordinate i (n-1)/(S-1)); the host twin of the warm start's
+    device interpolation."""
+    g = np.asarray(f4, np.float64).reshape(S, S, S, 4)
+    sc = np.arange(n) * (S - 1) / (n - 1)
+    k = np.minimum(S - 2, np.floor(sc).astype(int))
+    fr = sc - k
+    out = np.zeros((n, n, n, 4))
+    for dz in (0, 1):
+        wz = (fr if dz else 1 - fr)[:, None, None, None]
+        for dy in (0, 1):
+            wy = (fr if dy else 1 - fr)[None, :, None, None]
+            for dx in (0, 1):
+                wx = (fr if dx else 1 - fr)[None, None, :, None]
+                out += wz * wy * wx * g[np.ix_(k + dz, k + dy, k + dx)]
+    return out
+
+
+def equilibrium_moments(f, n, U):
+    """n^3 x 4 normalised fields -> the ten stored moments of the equilibrium
+    state (as lbm_sur_warm_start writes them: walls at rest, the lid row moving
+    at U along +x)."""
+    u, v, w, rm1 = U * f[..., 0], U * f[..., 1], U * f[..., 2], 3 * U * U * f[..., 3]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    wall = (x == 0) | (x == n - 1) | (y == 0) | (z == 0) | (z == n - 1)
+    lid = y == n - 1
+    u = np.where(lid, U, np.where(wall, 0.0, u))
+    v = np.where(lid | wall, 0.0, v)
+    w = np.where(lid | wall, 0.0, w)
+    m = [rm1, 3 * u, 3 * v, 3 * w, 4.5 * u * u, 9 * u * v, 9 * u * w, 4.5 * v * v, 9 * v * w, 4.5 * w * w]
+    return [a.astype(np.float32).reshape(-1) for a in m]
+
+
+def perfect_bound(cases, grids, tol=1e-6, check=100, max_steps=200000, log=print):
+    """What a perfect network on an S^3 grid could gain: each case solved cold to
+    tol, its solution sampled on the S^3 grid and interpolated back exactly as a
+    prediction is, and solved again to tol from there."""
+    out = []
+    for n, Re, U in cases:
+        with N.Solver(n, n, n, Re, U) as s:
+            cold = N.solve_to_residual(s, Re, U, tol, check, max_steps)
+            mc = s.get_moments()
+        row = {"N": n, "Re": Re, "U": U, "cold_steps": int(cold["steps"]), "steady": bool(cold["residual"] < tol)}
+        for S in grids:
+            with N.Solver(n, n, n, Re, U) as s:
+                s.set_moments(equilibrium_moments(upsample(normalised_fields(mc, n, U, S), S, n), n, U))
+                warm = N.solve_to_residual(s, Re, U, tol, check, max_steps)
+            row[f"S{S}"] = {"steps": int(warm["steps"]), "step_ratio": warm["steps"] / max(cold["steps"], 1),
+                            "first_residual": warm["first_residual"]}
+            log(f"# N {n} Re {Re:.0f}: cold {cold['steps']} steps; perfect S={S}: {warm['steps']} steps "
+                f"(ratio {warm['steps'] / max(cold['steps'], 1):.3f}, first r {warm['first_residual']:.3g})")
+        out.append(row)
+    return out
+
+
 def summarise(res):
     """Medians over the steady cases; unsteady ones counted separately."""
     st = [r for r in res if r["steady"]]
@@ -248,6 +312,8 @@ def main(argv=None):
     g.add_argument("--min-edge", type=int, default=0, help="keep only the set's cases with N >= this")
     g.add_argument("--max-edge", type=int, default=1 << 30, help="keep only the set's cases with N <= this")
     g.add_argument("--S", type=int, default=S_NET, help="network grid edge of the targets")
+    g.add_argument("--first", type=int, default=0, help="solve cases first .. first+count-1 of the (filtered) set")
+    g.add_argument("--count", type=int, default=0, help="number of cases of the slice (0: all)")
     g.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--out", required=True)
@@ -267,17 +333,27 @@ def main(argv=None):
     e.add_argument("--seed", type=int, default=HELDOUT_SEED)
     e.add_argument("--first", type=int, default=0, help="evaluate cases first .. first+count-1 of the draw")
     e.add_argument("--count", type=int, default=0, help="number of cases of the slice (0: all)")
+    e.add_argument("--indices", default="", help="comma-separated case indices of the draw (instead of a slice)")
     e.add_argument("--converge-tol", type=float, default=0.0,
                    help="continue each cold run to this residual and compare both fields with it (0: off)")
     e.add_argument("--converge-steps", type=int, default=200000)
     e.add_argument("--re-range", default="400,2500", help="Re interval of the log-uniform draw")
     e.add_argument("--json", default="")
+    bd = sub.add_parser("bound", help="perfect-network bound: converged fields interpolated from S^3 grids")
+    bd.add_argument("--edges", default="128,192")
+    bd.add_argument("--cases", type=int, default=64)
+    bd.add_argument("--seed", type=int, default=HELDOUT_SEED)
+    bd.add_argument("--re-range", default="400,2500")
+    bd.add_argument("--indices", required=True, help="comma-separated case indices of the draw")
+    bd.add_argument("--grids", default="64,96,128")
     a = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "gen":
         d = DATA_SETS[a.set]
-        cases = [c for c in draw_cases(d["cases"], d["seed"], d["edges"], edge_p=d["edge_p"])
+        cases = [c for c in draw_cases(d["cases"], d["seed"], d["edges"], edge_p=d["edge_p"],
+                                       re_range=d.get("re_range", (100.0, 3200.0)))
                  if a.min_edge <= c[0] <= a.max_edge]
+        cases = cases[a.first:a.first + a.count] if a.count else cases[a.first:]
         t0 = time.perf_counter()
         _, y, info = generate(cases, max_steps=d["max_steps"], log=log, S=a.S)
         save_data(a.out, y, info, S=a.S)
@@ -312,14 +388,30 @@ def main(argv=None):
                 "lr": a.lr}
         save_model(a.out, net, meta)
         print(json.dumps(meta), flush=True)
+    elif a.cmd == "bound":
+        edges = tuple(int(v) for v in a.edges.split(","))
+        lo, hi = (float(v) for v in a.re_range.split(","))
+        draw = draw_cases(a.cases, a.seed, edges, re_range=(lo, hi))
+        idx = [int(v) for v in a.indices.split(",")]
+        rows = perfect_bound([draw[i] for i in idx], [int(v) for v in a.grids.split(",")], log=log)
+        for i, r in zip(idx, rows):
+            r["index"] = i
+        print(json.dumps(rows), flush=True)
     else:
         net, meta = load_model(a.model)
         edges = tuple(int(v) for v in a.edges.split(","))
         lo, hi = (float(v) for v in a.re_range.split(","))
         cases = draw_cases(a.cases, a.seed, edges, re_range=(lo, hi))
-        cases = cases[a.first:a.first + a.count] if a.count else cases[a.first:]
+        if a.indices:
+            idx = [int(v) for v in a.indices.split(",")]
+            cases = [cases[i] for i in idx]
+        else:
+            idx = list(range(len(cases)))[a.first:a.first + a.count] if a.count else list(range(len(cases)))[a.first:]
+            cases = cases[a.first:a.first + a.count] if a.count else cases[a.first:]
         res = evaluate(net, cases, max_steps=a.max_steps, log=log,
                        converge_tol=a.converge_tol or None, converge_steps=a.converge_steps)
+        for i, r in zip(idx, res):
+            r["index"] = i
         summary = {**summarise(res), "seed": a.seed, "first": a.first, "edges": list(edges),
                    "re_range": [lo, hi], "model": os.path.basename(a.model), "per_case": res}
         if a.json:
