@@ -108,6 +108,37 @@ bool launch_fill_q(int tx, int ty, const StepParams &p, int grid, hipStream_t s,
     return true;
 }
 
+// Workgroups of one step-kernel instantiation resident per CU (0 if the runtime
+// cannot tell).
+template <int Q, int TX, int TY>
+int step_occupancy_t(int storage, bool fo) {
+    int nb = 0;
+    hipError_t e;
+    if (storage == LBM_STORE_FP32)
+        e = fo ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<Q, TX, TY, float, true, false>, TX * TY, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<Q, TX, TY, float, false, false>, TX * TY, 0);
+    else
+        e = fo ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<Q, TX, TY, _Float16, true, false>, TX * TY, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<Q, TX, TY, _Float16, false, false>, TX * TY, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
+}
+template <int Q>
+int step_occupancy_q(int tx, int ty, int storage, bool fo) {
+    if (tx == 16 && ty == 16) return step_occupancy_t<Q, 16, 16>(storage, fo);
+    if (tx == 32 && ty == 8) return step_occupancy_t<Q, 32, 8>(storage, fo);
+    if (tx == 8 && ty == 8) return step_occupancy_t<Q, 8, 8>(storage, fo);
+    if (tx == 32 && ty == 16) return step_occupancy_t<Q, 32, 16>(storage, fo);
+    if constexpr (Q == 19) {
+        if (tx == 32 && ty == 32) return step_occupancy_t<Q, 32, 32>(storage, fo);
+        if (tx == 64 && ty == 16) return step_occupancy_t<Q, 64, 16>(storage, fo);
+    }
+    return 0;
+}
+
 }  // namespace
 
 namespace lbm {
@@ -144,8 +175,9 @@ int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_
 }
 
 // Planes per work item.  A launch runs in rounds of `slots` resident
-// workgroups; a work item of lz planes costs about (lz + 4) plane times (the
-// 4: pipeline fill and drain, ring and z-face set-up) at bpn(lz) bytes per
+// workgroups; a work item of lz planes costs about (lz + h) plane times (h:
+// pipeline fill and drain, ring and z-face set-up; 4 for D3Q19, 8 for D3Q27
+// with its nine face slots per side) at bpn(lz) bytes per
 // node, bpn = M + 16 QF (1/TX + 1/TY + 1/lz) with M = 80 moment bytes (40 in
 // FP16) and the face words.  The choice minimises rounds x (lz + 4) x bpn(lz)
 // over the divisors of nz in [4, 64]; with FP32 storage a single round costs
@@ -159,6 +191,9 @@ int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_
 // 256^3 lz 32 (49.1k; 16: 46.6k, 64: 42.1k), 384^3 lz 48 (53.7k; 32: 52.7k).
 // With the FP64 means on (mean_bytes per node read + written), a single round
 // pays no penalty either: D3Q19 32x32 at 256^3 lz 64 (616 us; 32: 636, 16: 649).
+// D3Q27 32x16 (two workgroups per CU): 256^3 FP32 lz 64 (266.1 us; 32: 272.7),
+// FP16 lz 64 (180.0; 32: 182.8); 384^3 FP32 lz 48 (1054.9; 64: 1059.0, 32:
+// 1081.9), FP16 lz 48 (631.7; 64: 640.0, 32: 664.9) -- h = 4 would pick 32.
 int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool fp16, int mean_bytes) {
     const int64_t cols = (int64_t)(nx / tx) * (ny / ty);
     const double face = 16.0 * (Q == 19 ? 5 : 9);
@@ -171,7 +206,8 @@ int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool
         const int64_t items = cols * (nz / lz);
         const int64_t rounds = (items + slots - 1) / slots;
         const double bpn = mom + face * (1.0 / tx + 1.0 / ty + 1.0 / lz);
-        const double cost = (double)rounds * (lz + 4) * bpn * (rounds == 1 && single_round_penalty ? 1.1 : 1.0);
+        const double cost =
+            (double)rounds * (lz + (Q == 19 ? 4 : 8)) * bpn * (rounds == 1 && single_round_penalty ? 1.1 : 1.0);
         if (best < 0 || cost < best_cost) {
             best = lz;
             best_cost = cost;
@@ -183,14 +219,19 @@ int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool
     return nz;
 }
 
-// Workgroups resident at once: CUs x workgroups per CU at the step kernel's
-// occupancy (4 waves per SIMD for D3Q19, 3 for D3Q27; 4 SIMDs per CU).
-int64_t resident_workgroups(int Q, int tx, int ty, int device) {
+// Workgroups resident at once: CUs x workgroups per CU of the step kernel the
+// case will run (registers and LDS of that instantiation, from the runtime;
+// failing that, 4 waves per SIMD for D3Q19 and 3 for D3Q27, 4 SIMDs per CU).
+int64_t resident_workgroups(int Q, int tx, int ty, int device, int storage, bool fo) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
         cus = 256;  // MI355X
-    const int waves = (tx * ty + 63) / 64;
-    const int per_cu = std::max(1, (4 * (Q == 19 ? 4 : 3)) / waves);
+    if (hipSetDevice(device) != hipSuccess) (void)hipGetLastError();  // the occupancy query asks the current device
+    int per_cu = Q == 19 ? step_occupancy_q<19>(tx, ty, storage, fo) : step_occupancy_q<27>(tx, ty, storage, fo);
+    if (per_cu <= 0) {
+        const int waves = (tx * ty + 63) / 64;
+        per_cu = std::max(1, (4 * (Q == 19 ? 4 : 3)) / waves);
+    }
     return (int64_t)cus * per_cu;
 }
 
@@ -200,11 +241,14 @@ void choose_tile(int Q, int nx, int ny, bool fp16, bool means, int *tx, int *ty)
     // With the FP64 means on, the D3Q19 step runs 7% faster (FP32: 616 vs 661 us,
     // FP16: 554 vs 568 us) on 32x32 footprints, one 16-wave workgroup per CU
     // (without the means 32x32 is 4% slower: 272 vs 262 us).
-    // D3Q27 with FP16 storage runs 9% faster at 32x8 (256^3: 342 vs 373 us;
-    // 384^3: 1054 vs 1152 us); with FP32 storage 16x16 stays ahead at 384^3
-    // (1234 vs 1294 us; 256^3: 396 vs 390 us).
+    // D3Q27 without the means (one up-going LDS bank, two 32x16 workgroups per
+    // CU) is fastest on 32x16: 256^3 FP32 265 vs 350 us (16x16), FP16 180 vs
+    // 241 us (32x8); 384^3 FP32 1055 vs 1183 us, FP16 632 vs 794 us.  With the
+    // means (two banks) FP16 runs 9% faster at 32x8 than at 16x16 (256^3: 342 vs
+    // 373 us; 384^3: 1054 vs 1152 us).
     if (Q == 19 && means && nx % 32 == 0 && ny % 32 == 0) { *tx = 32; *ty = 32; }
     else if (Q == 19 && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
+    else if (Q == 27 && !means && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
     else if (Q == 27 && fp16 && nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
     else if (nx % 16 == 0 && ny % 16 == 0) { *tx = 16; *ty = 16; }
     else if (nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
@@ -223,7 +267,7 @@ void auto_tiling(const lbm_config &c, unsigned fo_mask, int *tx, int *ty, int *l
     const int mb = mean_bytes(fo_mask);
     choose_tile(c.velocity_set, c.nx, c.ny, fp16, mb > 0, tx, ty);
     *lz = choose_lz(c.velocity_set, c.nx, c.ny, c.nz, *tx, *ty,
-                    resident_workgroups(c.velocity_set, *tx, *ty, c.device), fp16, mb);
+                    resident_workgroups(c.velocity_set, *tx, *ty, c.device, c.storage, mb > 0), fp16, mb);
 }
 
 // Internal (column-tiled) index of a node.
@@ -743,7 +787,7 @@ int lbm_create(const lbm_config *cfg, lbm_handle **out) {
     if (h->n > ((int64_t)1 << 30)) return bad(LBM_ERR_SIZE, "mesh too large: at most 2^30 nodes (4 GiB per moment array)");
     h->LZ = c.tile_z > 0 ? c.tile_z
                          : choose_lz(h->Q, c.nx, c.ny, c.nz, h->TX, h->TY,
-                                     resident_workgroups(h->Q, h->TX, h->TY, c.device),
+                                     resident_workgroups(h->Q, h->TX, h->TY, c.device, c.storage, false),
                                      c.storage == LBM_STORE_FP16, 0);
     if (c.nz % h->LZ) return bad(LBM_ERR_SIZE, "nz must be a multiple of tile_z");
     h->nbx = c.nx / h->TX;

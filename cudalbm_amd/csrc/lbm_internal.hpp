@@ -43,7 +43,7 @@ void launch_fill(int Q, int tx, int ty, const StepParams &p, int grid, hipStream
 // Words of one face-buffer set and the offsets of its Y and Z blocks.
 int64_t face_words(int Q, int nx, int ny, int nz, int tx, int ty, int lz, int64_t *yoff, int64_t *zoff);
 int choose_lz(int Q, int nx, int ny, int nz, int tx, int ty, int64_t slots, bool fp16, int mean_bytes);
-int64_t resident_workgroups(int Q, int tx, int ty, int device);
+int64_t resident_workgroups(int Q, int tx, int ty, int device, int storage, bool fo);
 void choose_tile(int Q, int nx, int ny, bool fp16, bool means, int *tx, int *ty);
 std::vector<int64_t> make_perm(int layout, int nx, int ny, int nz, int tx, int ty);
 

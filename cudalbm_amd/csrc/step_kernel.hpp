@@ -118,6 +118,7 @@ __device__ __forceinline__ void *byte_at(void *p, unsigned off) { return static_
 template <>
 struct Store<float> {
     static constexpr unsigned kBytes = 4;
+    static constexpr int kWords = 10;  // 4-byte words of one node's moments
     __device__ static __forceinline__ float load(const void *p, unsigned off, float) {
         return __builtin_nontemporal_load(static_cast<const float *>(byte_at(p, off)));
     }
@@ -136,6 +137,7 @@ struct Store<float> {
 template <>
 struct Store<_Float16> {
     static constexpr unsigned kBytes = 4;
+    static constexpr int kWords = 5;  // 4-byte node words of one node's moments
     __device__ static __forceinline__ float load(const void *p, unsigned off, float dec) {
         return float(__builtin_nontemporal_load(static_cast<const _Float16 *>(byte_at(p, off)))) * dec;
     }
