@@ -236,17 +236,18 @@ int64_t resident_workgroups(int Q, int tx, int ty, int device, int storage, bool
 }
 
 void choose_tile(int Q, int nx, int ny, bool fp16, bool means, int *tx, int *ty) {
-    // measured on MI355X at 256^3: D3Q19 is fastest with 32x16 footprints (8 waves,
-    // fewest face words per node); D3Q27's LDS allows 16x16 at 3 workgroups/CU.
-    // With the FP64 means on, the D3Q19 step runs 7% faster (FP32: 616 vs 661 us,
-    // FP16: 554 vs 568 us) on 32x32 footprints, one 16-wave workgroup per CU
-    // (without the means 32x32 is 4% slower: 272 vs 262 us).
+    // measured on MI355X at 256^3: the D3Q19 FP32 step is fastest with 32x16
+    // footprints (8 waves, two workgroups per CU): 246.3 us against 249.8 on
+    // 32x32x64, 255.3 on 64x16x64.  With FP16 storage or the FP64 means on, 32x32
+    // (one 16-wave workgroup per CU, 6.25 instead of 10 face bytes per node) wins:
+    // FP16 132.9 vs 138.6 us (32x16x64), with the means FP32 616 vs 661 us and
+    // FP16 554 vs 568 us.
     // D3Q27 without the means (one up-going LDS bank, two 32x16 workgroups per
     // CU) is fastest on 32x16: 256^3 FP32 265 vs 350 us (16x16), FP16 180 vs
     // 241 us (32x8); 384^3 FP32 1055 vs 1183 us, FP16 632 vs 794 us.  With the
     // means (two banks) FP16 runs 9% faster at 32x8 than at 16x16 (256^3: 342 vs
     // 373 us; 384^3: 1054 vs 1152 us).
-    if (Q == 19 && means && nx % 32 == 0 && ny % 32 == 0) { *tx = 32; *ty = 32; }
+    if (Q == 19 && (means || fp16) && nx % 32 == 0 && ny % 32 == 0) { *tx = 32; *ty = 32; }
     else if (Q == 19 && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
     else if (Q == 27 && !means && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
     else if (Q == 27 && fp16 && nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }

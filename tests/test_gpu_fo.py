@@ -152,19 +152,27 @@ def test_fp16_node_words_round_trip(native_lib, Q):
         assert np.array_equal(again[k], want[k]), k
 
 
-@pytest.mark.parametrize("storage", ["fp32", "fp16"])
-def test_fo_enable_keeps_the_moments(native_lib, storage):
-    """Turning the FP64 means on re-tiles an automatically tiled D3Q19 case
-    (32x16 -> 32x32 footprints, moments permuted on the device, face buffers
-    rebuilt) and turning them off moves it back; the moments survive both moves
-    bit for bit, a run with both moves equals a run with none (results do not
-    depend on the tiling), and the residual's stored previous field follows the
-    moves (FP64 sums over another node order: 1e-12 relative)."""
+@pytest.mark.parametrize("Q,storage,plain,means", [
+    (19, "fp32", (32, 16), (32, 32)),
+    (19, "fp16", (32, 32), (32, 32)),
+    (27, "fp32", (32, 16), (16, 16)),
+    (27, "fp16", (32, 16), (32, 8)),
+])
+def test_fo_enable_keeps_the_moments(native_lib, Q, storage, plain, means):
+    """Turning the FP64 means on re-tiles an automatically tiled case whose
+    fastest footprint changes with them (D3Q19 FP32 32x16 -> 32x32; D3Q27
+    32x16 with one up-going LDS bank -> 16x16 or, FP16, 32x8 with two: moments
+    permuted on the device, FP16 node words included, face buffers rebuilt) and
+    turning them off moves it back; D3Q19 FP16 stays on 32x32.  The moments
+    survive both moves bit for bit, a run with both moves equals a run with
+    none (results do not depend on the tiling), and the residual's stored
+    previous field follows the moves (FP64 sums over another node order: 1e-12
+    relative)."""
     from cudalbm_amd import Solver
     from cudalbm_amd import native as N
 
     n = 32
-    with Solver(n, n, n, 400.0, 0.05, velocity_set=19, storage=storage) as s:
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=Q, storage=storage) as s:
         t0 = s.tiling
         s.step(30)
         s.reduce()
@@ -181,7 +189,7 @@ def test_fo_enable_keeps_the_moments(native_lib, storage):
         s.step(5)
         ke2, r2 = s.reduce()
         d = s.get_moments()
-    with Solver(n, n, n, 400.0, 0.05, velocity_set=19, storage=storage) as s:
+    with Solver(n, n, n, 400.0, 0.05, velocity_set=Q, storage=storage) as s:
         s.step(30)
         s.reduce()
         s.step(5)
@@ -189,8 +197,8 @@ def test_fo_enable_keeps_the_moments(native_lib, storage):
         s.step(5)
         ke2r, r2r = s.reduce()
         e = s.get_moments()
-    print(f"D3Q19 {storage}: tilings {t0} -> {t1} -> {t2}; moments kept across both moves", flush=True)
-    assert t1[:2] == (32, 32) and t0 == t2 and t0[:2] == (32, 16)
+    print(f"D3Q{Q} {storage}: tilings {t0} -> {t1} -> {t2}; moments kept across both moves", flush=True)
+    assert t0[:2] == plain and t1[:2] == means and t2 == t0
     for x, y in ((ke1, ke1r), (r1, r1r), (ke2, ke2r), (r2, r2r)):
         assert abs(x - y) <= 1e-12 * abs(y), (x, y)
     for k in range(10):
@@ -198,22 +206,6 @@ def test_fo_enable_keeps_the_moments(native_lib, storage):
         assert np.array_equal(c[k], c2[k]), k
         assert np.array_equal(d[k], e[k]), k
     assert not np.array_equal(c[1], d[1])
-
-
-def test_fo_enable_no_move_d3q27(native_lib):
-    """D3Q27 keeps one (shifted) layout whatever the function objects: enabling
-    the means leaves the moments untouched."""
-    from cudalbm_amd import Solver
-    from cudalbm_amd import native as N
-
-    n = 32
-    with Solver(n, n, n, 400.0, 0.05, velocity_set=27) as s:
-        s.step(20)
-        a = s.get_moments()
-        s.fo_enable(N.LBM_FO_KMEAN | N.LBM_FO_MOMENTSMEAN)
-        b = s.get_moments()
-    for k in range(10):
-        assert np.array_equal(a[k], b[k]), k
 
 
 @pytest.mark.parametrize("Q", [19, 27])
