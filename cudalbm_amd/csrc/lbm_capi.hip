@@ -86,10 +86,9 @@ bool launch_step_q(int tx, int ty, const StepParams &p, int grid, hipStream_t s,
     else if (tx == 32 && ty == 8) launch_step_t<Q, 32, 8>(p, grid, s, storage, fo, probe);
     else if (tx == 8 && ty == 8) launch_step_t<Q, 8, 8>(p, grid, s, storage, fo, probe);
     else if (tx == 32 && ty == 16) launch_step_t<Q, 32, 16>(p, grid, s, storage, fo, probe);
+    else if (tx == 32 && ty == 32) launch_step_t<Q, 32, 32>(p, grid, s, storage, fo, probe);
     else if constexpr (Q == 19) {
-        // 1024-thread footprints (one workgroup per CU): D3Q19 only
-        if (tx == 32 && ty == 32) launch_step_t<Q, 32, 32>(p, grid, s, storage, fo, probe);
-        else if (tx == 64 && ty == 16) launch_step_t<Q, 64, 16>(p, grid, s, storage, fo, probe);
+        if (tx == 64 && ty == 16) launch_step_t<Q, 64, 16>(p, grid, s, storage, fo, probe);
         else return false;
     } else return false;
     return true;
@@ -100,9 +99,9 @@ bool launch_fill_q(int tx, int ty, const StepParams &p, int grid, hipStream_t s,
     else if (tx == 32 && ty == 8) launch_fill_t<Q, 32, 8>(p, grid, s, storage, probe);
     else if (tx == 8 && ty == 8) launch_fill_t<Q, 8, 8>(p, grid, s, storage, probe);
     else if (tx == 32 && ty == 16) launch_fill_t<Q, 32, 16>(p, grid, s, storage, probe);
+    else if (tx == 32 && ty == 32) launch_fill_t<Q, 32, 32>(p, grid, s, storage, probe);
     else if constexpr (Q == 19) {
-        if (tx == 32 && ty == 32) launch_fill_t<Q, 32, 32>(p, grid, s, storage, probe);
-        else if (tx == 64 && ty == 16) launch_fill_t<Q, 64, 16>(p, grid, s, storage, probe);
+        if (tx == 64 && ty == 16) launch_fill_t<Q, 64, 16>(p, grid, s, storage, probe);
         else return false;
     } else return false;
     return true;
@@ -132,8 +131,8 @@ int step_occupancy_q(int tx, int ty, int storage, bool fo) {
     if (tx == 32 && ty == 8) return step_occupancy_t<Q, 32, 8>(storage, fo);
     if (tx == 8 && ty == 8) return step_occupancy_t<Q, 8, 8>(storage, fo);
     if (tx == 32 && ty == 16) return step_occupancy_t<Q, 32, 16>(storage, fo);
+    if (tx == 32 && ty == 32) return step_occupancy_t<Q, 32, 32>(storage, fo);
     if constexpr (Q == 19) {
-        if (tx == 32 && ty == 32) return step_occupancy_t<Q, 32, 32>(storage, fo);
         if (tx == 64 && ty == 16) return step_occupancy_t<Q, 64, 16>(storage, fo);
     }
     return 0;
@@ -145,7 +144,7 @@ namespace lbm {
 
 bool tile_supported(int Q, int tx, int ty) {
     return (tx == 16 && ty == 16) || (tx == 32 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 16) ||
-           (Q == 19 && ((tx == 32 && ty == 32) || (tx == 64 && ty == 16)));
+           (tx == 32 && ty == 32) || (Q == 19 && tx == 64 && ty == 16);
 }
 
 void launch_step(int Q, int tx, int ty, const StepParams &p, int grid, hipStream_t s, int storage, bool fo,
@@ -242,13 +241,16 @@ void choose_tile(int Q, int nx, int ny, bool fp16, bool means, int *tx, int *ty)
     // (one 16-wave workgroup per CU, 6.25 instead of 10 face bytes per node) wins:
     // FP16 132.9 vs 138.6 us (32x16x64), with the means FP32 616 vs 661 us and
     // FP16 554 vs 568 us.
-    // D3Q27 without the means (one up-going LDS bank, two 32x16 workgroups per
-    // CU) is fastest on 32x16: 256^3 FP32 265 vs 350 us (16x16), FP16 180 vs
-    // 241 us (32x8); 384^3 FP32 1055 vs 1183 us, FP16 632 vs 794 us.  With the
-    // means (two banks) FP16 runs 9% faster at 32x8 than at 16x16 (256^3: 342 vs
-    // 373 us; 384^3: 1054 vs 1152 us).
+    // D3Q27 without the means (one up-going LDS bank) with FP32 storage is
+    // fastest on 32x16 (two workgroups per CU): 256^3 265 vs 350 us (16x16) and
+    // 272 us (32x32x64); 384^3 1055 vs 1183 us (32x32x32: 996 us).  With FP16
+    // storage the instruction-bound step gains from 32x32 (one workgroup per CU,
+    // fewer edge nodes to flush per node): 256^3 170.4 vs 180.7 us (32x16),
+    // 241 us (32x8); 384^3 628 vs 632 us.  With the means (two banks) FP16 runs
+    // 9% faster at 32x8 than at 16x16 (256^3: 342 vs 373 us; 384^3: 1054 vs 1152 us).
     if (Q == 19 && (means || fp16) && nx % 32 == 0 && ny % 32 == 0) { *tx = 32; *ty = 32; }
     else if (Q == 19 && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
+    else if (Q == 27 && !means && fp16 && nx % 32 == 0 && ny % 32 == 0) { *tx = 32; *ty = 32; }
     else if (Q == 27 && !means && nx % 32 == 0 && ny % 16 == 0) { *tx = 32; *ty = 16; }
     else if (Q == 27 && fp16 && nx % 32 == 0 && ny % 8 == 0) { *tx = 32; *ty = 8; }
     else if (nx % 16 == 0 && ny % 16 == 0) { *tx = 16; *ty = 16; }

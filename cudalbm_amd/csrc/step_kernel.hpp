@@ -216,9 +216,10 @@ static_assert(lds_slot<27>(9) == 16 && lds_slot<27>(25) == 23 && lds_slot<27>(26
 // each; `lds` = -1 when the thread has no job), the face-set offset without the
 // plane term (-1 when outside the domain), and plane stride * 4 + (dz + 1).
 struct Job {
-    int lds;  // bank0 | bank1 << 16, or -1
+    int lds;  // bank0 | bank1 << 16 (16-bit LDS word addresses, so the sign bit may be set), or -1: no job
     int gb;
     int zsd;
+    __device__ __forceinline__ bool valid() const { return lds != -1; }
     __device__ __forceinline__ int lds0() const { return lds & 0xFFFF; }
     __device__ __forceinline__ int lds1() const { return (lds >> 16) & 0xFFFF; }
     __device__ __forceinline__ int zs() const { return zsd >> 2; }
@@ -249,6 +250,8 @@ template <int Q, int TX, int TY>
 __device__ __forceinline__ Job make_job(int j, const StepParams &p, int bx, int by) {
     using G = Geo<TX, TY>;
     constexpr int QF = VS<Q>::QF;
+    // both banks' word addresses below 0xFFFF: a packed pair never reads as -1
+    static_assert((VS<Q>::NSLOT + VS<Q>::NU) * G::P < 0xFFFF, "ring-job LDS addresses fit 16 bits");
     Job job{-1, -1, 0};
     int start = 0;
     unroll<0, 4>([&](auto sc) {

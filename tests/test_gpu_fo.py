@@ -156,12 +156,12 @@ def test_fp16_node_words_round_trip(native_lib, Q):
     (19, "fp32", (32, 16), (32, 32)),
     (19, "fp16", (32, 32), (32, 32)),
     (27, "fp32", (32, 16), (16, 16)),
-    (27, "fp16", (32, 16), (32, 8)),
+    (27, "fp16", (32, 32), (32, 8)),
 ])
 def test_fo_enable_keeps_the_moments(native_lib, Q, storage, plain, means):
     """Turning the FP64 means on re-tiles an automatically tiled case whose
     fastest footprint changes with them (D3Q19 FP32 32x16 -> 32x32; D3Q27
-    32x16 with one up-going LDS bank -> 16x16 or, FP16, 32x8 with two: moments
+    32x16 or, FP16, 32x32 with one up-going LDS bank -> 16x16 or 32x8 with two: moments
     permuted on the device, FP16 node words included, face buffers rebuilt) and
     turning them off moves it back; D3Q19 FP16 stays on 32x32.  The moments
     survive both moves bit for bit, a run with both moves equals a run with

@@ -39,6 +39,7 @@ def _expected_sources(Q, nx, ny, nz):
     ((64, 16, 16), (32, 8, 4)),
     ((24, 16, 8), (8, 8, 2)),
     ((64, 32, 16), (32, 16, 16)),
+    ((64, 64, 16), (32, 32, 8)),
 ])
 def test_streaming_probe_bit_exact(native_lib, Q, dims, tile):
     from cudalbm_amd import Solver
@@ -169,15 +170,15 @@ def test_parity_d3q27_128_re1000_fp32_fp16(native_lib):
     (19, "fp32", (32, 16, 32)),
     (19, "fp16", (32, 32, 64)),
     (27, "fp32", (32, 16, 64)),
-    (27, "fp16", (32, 16, 64)),
+    (27, "fp16", (32, 32, 64)),
 ])
 def test_automatic_tiling_256(native_lib, Q, storage, expected):
     """The automatic work decomposition at 256^3: the footprint and planes per
     work item that measured fastest for each velocity set and storage (D3Q19
-    FP16 on 32x32, no single-round penalty; D3Q27 on 32x16 with one up-going LDS bank, two
-    workgroups per CU).  The footprint does not depend on the device; the
-    planes per item do (resident workgroups), so they are checked on a 256-CU
-    device only."""
+    FP16 on 32x32, no single-round penalty; D3Q27 with one up-going LDS bank,
+    FP32 on 32x16 at two workgroups per CU, FP16 on 32x32).  The footprint
+    does not depend on the device; the planes per item do (resident
+    workgroups), so they are checked on a 256-CU device only."""
     import ctypes
 
     from cudalbm_amd import Solver
