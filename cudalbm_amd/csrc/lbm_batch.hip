@@ -501,6 +501,9 @@ double time_batch(lbm_batch *b, int steps) {
 }
 
 const int kTiles[][2] = {{32, 16}, {16, 16}, {32, 8}, {8, 8}};
+// single-case footprints: also the 1024-thread ones of the single-case kernel
+// (lbm_create rejects the combinations a velocity set does not build)
+const int kCaseTiles[][2] = {{32, 32}, {64, 16}, {32, 16}, {16, 16}, {32, 8}, {8, 8}};
 const int kLzTargets[] = {8, 16, 32, 64};
 
 }  // namespace
@@ -799,7 +802,7 @@ int lbm_autotune(const lbm_config *cfg, int steps, int *tile_x, int *tile_y, int
     if (!cfg || steps <= 0) return LBM_ERR_ARG;
     double best = INFINITY;
     int bt[3] = {0, 0, 0};
-    for (const auto &t : kTiles) {
+    for (const auto &t : kCaseTiles) {
         if (cfg->nx % t[0] || cfg->ny % t[1]) continue;
         std::vector<int> lzs;
         for (int lzt : kLzTargets) {

@@ -241,9 +241,12 @@ void choose_tile(int Q, int nx, int ny, bool fp16, bool means, int *tx, int *ty)
     // (one 16-wave workgroup per CU, 6.25 instead of 10 face bytes per node) wins:
     // FP16 132.9 vs 138.6 us (32x16x64), with the means FP32 616 vs 661 us and
     // FP16 554 vs 568 us.
-    // D3Q27 without the means (one up-going LDS bank) with FP32 storage is
-    // fastest on 32x16 (two workgroups per CU): 256^3 265 vs 350 us (16x16) and
-    // 272 us (32x32x64); 384^3 1055 vs 1183 us (32x32x32: 996 us).  With FP16
+    // D3Q27 without the means (one up-going LDS bank) with FP32 storage runs
+    // 32x16 (two workgroups per CU) unless its work items span more than one
+    // round of resident workgroups with a partial last one, then 32x32
+    // (auto_tiling): 256^3 265 us on 32x16x64 (512 items, one round) vs 272 us on
+    // 32x32x64 and 350 on 16x16; 384^3 996 us on 32x32x32 vs 1055 on 32x16x48
+    // (4.5 rounds) and 1183 on 16x16.  With FP16
     // storage the instruction-bound step gains from 32x32 (one workgroup per CU,
     // fewer edge nodes to flush per node): 256^3 170.4 vs 180.7 us (32x16),
     // 241 us (32x8); 384^3 628 vs 632 us.  With the means (two banks) FP16 runs
@@ -269,8 +272,19 @@ void auto_tiling(const lbm_config &c, unsigned fo_mask, int *tx, int *ty, int *l
     const bool fp16 = c.storage == LBM_STORE_FP16;
     const int mb = mean_bytes(fo_mask);
     choose_tile(c.velocity_set, c.nx, c.ny, fp16, mb > 0, tx, ty);
-    *lz = choose_lz(c.velocity_set, c.nx, c.ny, c.nz, *tx, *ty,
-                    resident_workgroups(c.velocity_set, *tx, *ty, c.device, c.storage, mb > 0), fp16, mb);
+    int64_t slots = resident_workgroups(c.velocity_set, *tx, *ty, c.device, c.storage, mb > 0);
+    *lz = choose_lz(c.velocity_set, c.nx, c.ny, c.nz, *tx, *ty, slots, fp16, mb);
+    // D3Q27 FP32 on 32x16 over more than one round of resident workgroups with a
+    // partial last one: 32x32 (one workgroup per CU) measured faster (384^3:
+    // 996 vs 1055 us); in whole rounds 32x16 stays ahead (256^3: 265 vs 272 us)
+    // and below one round it has twice the work items to spread
+    const int64_t items = (int64_t)(c.nx / *tx) * (c.ny / *ty) * (c.nz / *lz);
+    if (c.velocity_set == 27 && !fp16 && mb == 0 && *tx == 32 && *ty == 16 && c.ny % 32 == 0 && items > slots &&
+        items % slots != 0) {
+        *ty = 32;
+        slots = resident_workgroups(27, *tx, *ty, c.device, c.storage, false);
+        *lz = choose_lz(27, c.nx, c.ny, c.nz, *tx, *ty, slots, false, 0);
+    }
 }
 
 // Internal (column-tiled) index of a node.

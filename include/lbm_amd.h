@@ -128,8 +128,10 @@ int lbm_set_time(lbm_handle *h, int64_t t);
  * Enabling resets the means to zero.  Synchronizes the handle's stream.  Turning
  * the means on or off may move the moment arrays inside device memory, keeping
  * their values: with an automatic tiling (tile fields 0 at lbm_create) the case
- * is re-tiled for the step with or without the means (D3Q19 with the means: a
- * 32x32 footprint, 7% faster), otherwise only the arrays' spacing changes.  The
+ * is re-tiled for the step with or without the means when its fastest
+ * footprint differs (D3Q19 FP32: 32x16 without, 32x32 with the means; D3Q27:
+ * 32x16 or 32x32 with one up-going LDS bank without, 16x16 or 32x8 with two
+ * banks with the means), otherwise only the arrays' spacing changes.  The
  * move briefly holds the moments (and, re-tiling, the face buffers) twice; when
  * that memory is not available the case stays as it is.  Results do not depend
  * on the tiling (bit for bit).  LBM_ERR_NOMEM when an FO array cannot be

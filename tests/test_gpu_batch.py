@@ -160,7 +160,7 @@ def test_single_case_autotune(native_lib):
     from cudalbm_amd import Solver, autotune
 
     (tx, ty, tz), ms = autotune(64, 64, 64, 400.0, 0.05, steps=10)
-    assert (tx, ty) in [(32, 16), (16, 16), (32, 8), (8, 8)] and 64 % tz == 0 and ms > 0
+    assert (tx, ty) in [(32, 32), (64, 16), (32, 16), (16, 16), (32, 8), (8, 8)] and 64 % tz == 0 and ms > 0
     with Solver(64, 64, 64, 400.0, 0.05, tile=(tx, ty, tz)) as s:
         assert s.tiling == (tx, ty, tz)
 

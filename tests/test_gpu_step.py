@@ -176,9 +176,10 @@ def test_automatic_tiling_256(native_lib, Q, storage, expected):
     """The automatic work decomposition at 256^3: the footprint and planes per
     work item that measured fastest for each velocity set and storage (D3Q19
     FP16 on 32x32, no single-round penalty; D3Q27 with one up-going LDS bank,
-    FP32 on 32x16 at two workgroups per CU, FP16 on 32x32).  The footprint
-    does not depend on the device; the planes per item do (resident
-    workgroups), so they are checked on a 256-CU device only."""
+    FP32 on 32x16 at two workgroups per CU, FP16 on 32x32).  The planes per
+    item depend on the device (resident workgroups), and so does the D3Q27 FP32
+    footprint (32x32 when 32x16's work items leave a partial last round), so
+    those are checked on a 256-CU device only."""
     import ctypes
 
     from cudalbm_amd import Solver
@@ -191,6 +192,7 @@ def test_automatic_tiling_256(native_lib, Q, storage, expected):
     with Solver(256, 256, 256, 1000.0, 0.05, velocity_set=Q, storage=storage) as s:
         got = s.tiling
     print(f"D3Q{Q} {storage} 256^3 on {cus} CUs: tiling {got}", flush=True)
-    assert got[:2] == expected[:2]
     if cus == 256:
-        assert got[2] == expected[2]
+        assert got == expected
+    elif not (Q == 27 and storage == "fp32"):
+        assert got[:2] == expected[:2]
