@@ -114,10 +114,10 @@ def test_parity_tilings_d3q19(native_lib, tile):
 
 
 @pytest.mark.parametrize("Q,storage,tiles", [
-    (19, "fp32", [(32, 16, 32), (16, 16, 8), (8, 8, 4)]),
-    (19, "fp16", [(32, 16, 32), (16, 16, 8), (8, 8, 4)]),
-    (27, "fp32", [(16, 16, 32), (8, 8, 4)]),
-    (27, "fp16", [(16, 16, 32), (8, 8, 4)]),
+    (19, "fp32", [(32, 16, 32), (32, 32, 16), (16, 16, 8), (8, 8, 4)]),
+    (19, "fp16", [(32, 32, 32), (32, 16, 32), (16, 16, 8), (8, 8, 4)]),
+    (27, "fp32", [(32, 16, 32), (32, 32, 16), (16, 16, 32), (8, 8, 4)]),
+    (27, "fp16", [(32, 32, 32), (32, 16, 16), (16, 16, 32), (8, 8, 4)]),
 ])
 def test_determinism_runs_and_tilings(native_lib, Q, storage, tiles):
     """A repeat run at the same tiling is bit-identical, and so are runs at
