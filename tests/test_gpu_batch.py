@@ -24,11 +24,11 @@ def lz_for(nz, target):
     return 1
 
 
-def single(case, steps, tile):
+def single(case, steps, tile, Q=19):
     from cudalbm_amd import Solver
 
     nx, ny, nz, Re, u = case
-    with Solver(nx, ny, nz, Re, u, tile=(tile[0], tile[1], lz_for(nz, tile[2]))) as s:
+    with Solver(nx, ny, nz, Re, u, velocity_set=Q, tile=(tile[0], tile[1], lz_for(nz, tile[2]))) as s:
         s.step(steps)
         return s.get_moments()
 
@@ -54,6 +54,22 @@ def test_batch_matches_single_cases(native_lib, persistent, steps):
         assert list(b.steps) == [steps] * len(CASES)
     for i, c in enumerate(CASES):
         assert_same(got[i], single(c, steps, tile), f"case {i} {c}")
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_matches_single_cases_d3q27(native_lib, persistent):
+    """D3Q27 batches (one up-going LDS bank, the persistent form looping over
+    items with it) against single-case D3Q27 runs, bit for bit."""
+    from cudalbm_amd import Batch
+
+    tile, steps = (16, 16, 16), 41
+    print(f"D3Q27 batch persistent={persistent} steps={steps}", flush=True)
+    with Batch(CASES, velocity_set=27, tile=tile, persistent=persistent) as b:
+        assert b.tiling == (16, 16, 16, persistent)
+        b.step(steps)
+        got = [b.get_moments(i) for i in range(len(CASES))]
+    for i, c in enumerate(CASES):
+        assert_same(got[i], single(c, steps, tile, Q=27), f"D3Q27 case {i} {c}")
 
 
 def test_batch_case_against_oracle(native_lib):

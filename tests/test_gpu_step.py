@@ -166,15 +166,18 @@ def test_parity_d3q27_128_re1000_fp32_fp16(native_lib):
         assert er <= bound_rho
 
 
-@pytest.mark.parametrize("Q,storage,expected", [
-    (19, "fp32", (32, 16, 32)),
-    (19, "fp16", (32, 32, 64)),
-    (27, "fp32", (32, 16, 64)),
-    (27, "fp16", (32, 32, 64)),
+@pytest.mark.parametrize("n,Q,storage,expected", [
+    (256, 19, "fp32", (32, 16, 32)),
+    (256, 19, "fp16", (32, 32, 64)),
+    (256, 27, "fp32", (32, 16, 64)),
+    (256, 27, "fp16", (32, 32, 64)),
+    (384, 27, "fp32", (32, 32, 48)),
+    (384, 27, "fp16", (32, 32, 48)),
 ])
-def test_automatic_tiling_256(native_lib, Q, storage, expected):
-    """The automatic work decomposition at 256^3: the footprint and planes per
-    work item that measured fastest for each velocity set and storage (D3Q19
+def test_automatic_tiling(native_lib, n, Q, storage, expected):
+    """The automatic work decomposition at 256^3 and (D3Q27) 384^3: the
+    footprint and planes per work item that measured fastest for each velocity
+    set, storage and size (D3Q19
     FP16 on 32x32, no single-round penalty; D3Q27 with one up-going LDS bank,
     FP32 on 32x16 at two workgroups per CU, FP16 on 32x32).  The planes per
     item depend on the device (resident workgroups), and so does the D3Q27 FP32
@@ -189,9 +192,9 @@ def test_automatic_tiling_256(native_lib, Q, storage, expected):
     cus = ctypes.c_int(0)
     assert hip.hipDeviceGetAttribute(ctypes.byref(cus), 63, 0) == 0
     cus = cus.value
-    with Solver(256, 256, 256, 1000.0, 0.05, velocity_set=Q, storage=storage) as s:
+    with Solver(n, n, n, 1000.0, 0.05, velocity_set=Q, storage=storage) as s:
         got = s.tiling
-    print(f"D3Q{Q} {storage} 256^3 on {cus} CUs: tiling {got}", flush=True)
+    print(f"D3Q{Q} {storage} {n}^3 on {cus} CUs: tiling {got}", flush=True)
     if cus == 256:
         assert got == expected
     elif not (Q == 27 and storage == "fp32"):
