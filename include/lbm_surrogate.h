@@ -80,7 +80,7 @@ typedef struct lbm_solve_info {
 /* Runs a solver (at its current state when s is NULL, e.g. rest) until the
  * velocity residual r = ||u_t - u_{t-check}|| / ||u_t|| (FP64, lbm_reduce)
  * drops below tol or max_steps steps were taken.  With s, the solver is first
- * warm-started from the network; the warm start is judged at the first check:
+ * warm-started from the network; the warm start is tested at the first check:
  * a NaN or a residual >= 1 (what a start from rest shows at its first check,
  * since u_0 = 0) rejects it, the moments are reset to rest and the run
  * continues from there (the discarded steps still count in steps and ms). */

@@ -4,7 +4,7 @@ start, function objects k, kMean, S and SMean on, 1000 steps.
 
 The oracle's run (about 10 minutes on 8 cores) is precomputed by
 tests/golden/make_cavity256_d3q19.py into a fixture of sampled nodes and
-full-field norms.  Tolerances (north_star parity bar 1e-5 rel-L2):
+full-field norms.  Tolerances (the FP32 step's parity bar, 1e-5 rel-L2):
   * rho-1, u, v, w and the velocity vector: rel-L2 over the sampled nodes
     <= 1e-5 (8192 uniform samples estimate the full-field rel-L2; plus the
     nodes around the lid corner and one lid row);
